@@ -1,0 +1,116 @@
+/*
+ * libpq_hip -- MI355X (gfx950) backend for PicoQuant's pairwise tensor
+ * contraction path.  Plain C ABI: opaque context, integer tensor handles,
+ * plain pointers and sizes.  Every function returns 0 (PQ_OK) on success or a
+ * PQ_ERR_* code; pq_last_error() returns the message of the last failure on
+ * that context.
+ *
+ * Conventions (src/backends/interactive.jl, src/layer1.jl):
+ *   - tensors are column-major; dims are int64 and ordered as in the Node
+ *   - element type is fixed per context (InteractiveBackend{T}'s T):
+ *     PQ_C128 = ComplexF64, PQ_C64 = ComplexF32
+ *   - host buffers are interleaved (re, im), contiguous, column-major
+ *   - one context owns one HIP stream; calls on a context are not re-entrant
+ *   - device work is stream-ordered; pq_tensor_download synchronises
+ */
+#ifndef PQ_HIP_H
+#define PQ_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PQ_OK 0
+#define PQ_ERR_ARG 1          /* invalid argument (shapes, labels, handles) */
+#define PQ_ERR_HIP 2          /* HIP runtime failure */
+#define PQ_ERR_NOT_FOUND 3    /* unknown tensor handle (KeyError in the reference) */
+#define PQ_ERR_OOM 4          /* device allocation failed */
+#define PQ_ERR_UNSUPPORTED 5  /* shape outside this build's limits */
+
+#define PQ_C128 0
+#define PQ_C64 1
+
+#define PQ_MAX_RANK 64
+
+typedef struct pq_ctx pq_ctx;
+typedef int64_t pq_tensor; /* > 0 for a live tensor */
+
+/* Context = one backend instance: InteractiveBackend{T}() (src/backends/interactive.jl:10-23). */
+int pq_ctx_create(int device, int dtype, pq_ctx** out);
+int pq_ctx_destroy(pq_ctx* ctx);
+const char* pq_last_error(const pq_ctx* ctx);
+int pq_ctx_dtype(const pq_ctx* ctx);
+int pq_synchronize(pq_ctx* ctx);
+
+/* save_tensor_data: store a converted copy (src/backends/interactive.jl:32-36).
+ * host_dtype (PQ_C128 / PQ_C64) is converted to the context's element type. */
+int pq_tensor_upload(pq_ctx* ctx, int rank, const int64_t* dims, const void* host,
+                     int host_dtype, pq_tensor* out);
+
+/* Shape query with no data transfer (partition_network_on_virtual_bonds reads
+ * only the shape, src/layer2/slicing.jl:46-48). dims must hold PQ_MAX_RANK. */
+int pq_tensor_shape(pq_ctx* ctx, pq_tensor t, int* rank, int64_t* dims);
+
+/* load_tensor_data (src/backends/interactive.jl:44-49): copy out, column-major
+ * contiguous, in the context's element type.  Synchronising. */
+int pq_tensor_download(pq_ctx* ctx, pq_tensor t, void* host);
+
+/* delete_tensor! (src/backends/interactive.jl:159-161). */
+int pq_tensor_free(pq_ctx* ctx, pq_tensor t);
+
+/* contract_tensors (src/backends/interactive.jl:60-75 -> src/layer1.jl:323-330):
+ * ncon labels, positive = summed (each appears once in A and once in B),
+ * negative = open; C's index order is symdiff(ia, ib): A's open indices in A's
+ * order, then B's open indices in B's order.  A and B are not consumed; the
+ * caller frees them (the reference deletes both after the contraction). */
+int pq_contract(pq_ctx* ctx, pq_tensor A, int rankA, const int32_t* ia,
+                pq_tensor B, int rankB, const int32_t* ib, pq_tensor* C);
+
+/* permute_tensor (src/backends/interactive.jl:111-115 -> src/layer1.jl:349-352):
+ * perm is 1-based as in permutedims.  Zero-copy: the result is a strided view
+ * that consumers read through; download/reshape materialise it. */
+int pq_permute(pq_ctx* ctx, pq_tensor t, const int32_t* perm, pq_tensor* out);
+
+/* reshape_tensor (src/backends/interactive.jl:97-102): new dims, same element
+ * count, column-major; in place on the handle (metadata only when contiguous). */
+int pq_reshape(pq_ctx* ctx, pq_tensor t, int rank, const int64_t* dims);
+
+/* view_tensor! -> tensor_view (src/backends/interactive.jl:169-172,
+ * src/layer1.jl:429-432): restrict 1-based axis to lo:hi (inclusive); the
+ * axis is kept with extent hi-lo+1.  Zero-copy. */
+int pq_view(pq_ctx* ctx, pq_tensor t, int axis, int64_t lo, int64_t hi, pq_tensor* out);
+
+/* Materialise a strided tensor into a new contiguous one (used by save_output
+ * and by callers that need dense storage). */
+int pq_materialize(pq_ctx* ctx, pq_tensor t, pq_tensor* out);
+
+/* Device pointer and element offset of a tensor (for in-process callers that
+ * share device memory, e.g. tests comparing against torch). */
+int pq_tensor_device_ptr(pq_ctx* ctx, pq_tensor t, void** base, int64_t* elem_offset,
+                         int64_t* strides);
+
+/* Per-kernel-class launch timing with HIP events on the context's stream.
+ * enable=1 starts recording (clears totals), 0 stops.  Classes: see PQ_KC_*. */
+#define PQ_KC_STREAM 0   /* big x small operand (bandwidth-bound) */
+#define PQ_KC_GEMM 1     /* tiled complex GEMM on MFMA */
+#define PQ_KC_GENERIC 2  /* general gathered contraction */
+#define PQ_KC_REDUCE 3   /* split-K reduction for tiny outputs */
+#define PQ_KC_COPY 4     /* strided copy / permutation */
+#define PQ_KC_BATCHED 5  /* batched small contractions (one group per problem) */
+#define PQ_KC_ACCUM 6    /* FP64 cross-slice accumulation */
+#define PQ_KC_COUNT 8
+int pq_profile_enable(pq_ctx* ctx, int enable);
+int pq_profile_read(pq_ctx* ctx, int kclass, int64_t* launches, double* total_ms,
+                    double* alg_bytes, double* alg_flops);
+
+/* Kernel selection override for tests and the autotuner: -1 = automatic,
+ * otherwise force PQ_KC_STREAM / PQ_KC_GEMM / PQ_KC_GENERIC / PQ_KC_REDUCE
+ * when the shape allows it (PQ_ERR_UNSUPPORTED otherwise). */
+int pq_set_kernel_override(pq_ctx* ctx, int kclass);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PQ_HIP_H */

@@ -1,0 +1,51 @@
+// Internal interface between the runtime (context, tensors, allocator) and
+// the kernel launchers.
+#pragma once
+#include "pq_common.h"
+#include <string>
+
+// Big x small pairwise contraction: X is the large operand, Y the small one
+// (at most PQ_STREAM_KMAX contracted elements, PQ_STREAM_YMAX elements).
+//   M: X-open  (s0 = stride in X, s1 = stride in C), fastest C stride first
+//   K: summed  (s0 = stride in X, s1 = stride in Y)
+//   N: Y-open  (s0 = stride in Y, s1 = stride in C)
+#define PQ_STREAM_KMAX 32
+#define PQ_STREAM_YMAX 2048
+
+struct StreamArgs {
+  Group M, K, N;
+};
+
+// General contraction (any shape):
+//   M: A-open (s0 = A, s1 = C); N: B-open (s0 = B, s1 = C); K: (s0 = A, s1 = B)
+struct GenArgs {
+  Group M, N, K;
+};
+
+// Strided copy: group over output linear order, s0 = input stride, s1 = output stride.
+struct CopyArgs {
+  Group G;
+};
+
+// Tiled GEMM on MFMA with gathered operand loads and scattered C stores.
+// Offsets are split as (row group, col group) so a tile loads through two
+// small tables instead of per-element decoding.
+struct GemmArgs {
+  Group M, N, K;      // as GenArgs
+};
+
+template <typename C>
+hipError_t launch_stream(const C* X, const C* Y, C* Out, const StreamArgs& a, hipStream_t s);
+template <typename C>
+hipError_t launch_generic(const C* A, const C* B, C* Out, const GenArgs& a, hipStream_t s);
+template <typename C>
+hipError_t launch_reduce(const C* A, const C* B, C* Out, const GenArgs& a, C* partials,
+                         int nblocks, hipStream_t s);
+template <typename C>
+hipError_t launch_copy(const C* In, C* Out, const CopyArgs& a, hipStream_t s);
+template <typename C>
+hipError_t launch_gemm(const C* A, const C* B, C* Out, const GemmArgs& a, int32_t* tables,
+                       hipStream_t s);
+
+// Number of partial blocks the split-K reduction uses for a given K extent.
+int reduce_blocks_for(uint32_t ktotal);

@@ -1,0 +1,234 @@
+// Bandwidth-bound contraction kernels for gfx950 (64-wide wavefronts).
+//
+//  k_stream  : big x small pairwise contraction (gate application, absorbing
+//              a rank-1 output tensor, outer products with a small factor).
+//              One lane owns one X-open index; the small operand Y sits in LDS
+//              as a dense [K][N] table, so the big operand is read once and C
+//              written once -- the HBM roofline for bond-dimension-2 steps.
+//  k_generic : any shape; one lane per output element, loop over K.
+//  k_reduce  : tiny output (M*N <= 16) with a long summed extent: split-K
+//              partial sums per workgroup, then a deterministic second pass.
+//  k_copy    : strided gather into contiguous column-major storage
+//              (materialises permuted / sliced views).
+#include "pq_internal.h"
+
+namespace {
+
+template <typename C>
+__device__ __forceinline__ C czero() { C r; r.x = 0; r.y = 0; return r; }
+
+template <typename C>
+__device__ __forceinline__ C ldg(const C* p) { return *p; }
+
+// ---------------------------------------------------------------- k_stream
+template <typename C, int KP>
+__global__ __launch_bounds__(256) void k_stream(const C* __restrict__ X, const C* __restrict__ Y,
+                                                C* __restrict__ Out, StreamArgs a) {
+  extern __shared__ unsigned char smem_raw[];
+  const uint32_t Kt = a.K.total, Nt = a.N.total;
+  C* ys = reinterpret_cast<C*>(smem_raw);                              // [KP][Nt]
+  int32_t* xk = reinterpret_cast<int32_t*>(ys + (size_t)KP * Nt);      // [KP]
+  int32_t* cn = xk + KP;                                               // [Nt]
+  for (uint32_t t = threadIdx.x; t < KP * Nt; t += blockDim.x) {
+    uint32_t k = t % KP, n = t / KP;
+    C v = czero<C>();
+    if (k < Kt) {
+      int32_t ox, oyk, oyn, ocn;
+      group_offsets(a.K, k, ox, oyk);
+      group_offsets(a.N, n, oyn, ocn);
+      v = Y[oyk + oyn];
+    }
+    ys[(size_t)n * KP + k] = v;
+  }
+  for (uint32_t t = threadIdx.x; t < KP; t += blockDim.x) {
+    int32_t ox = 0, oy = 0;
+    if (t < Kt) group_offsets(a.K, t, ox, oy);
+    xk[t] = ox;
+  }
+  for (uint32_t t = threadIdx.x; t < Nt; t += blockDim.x) {
+    int32_t oy, oc;
+    group_offsets(a.N, t, oy, oc);
+    cn[t] = oc;
+  }
+  __syncthreads();
+  const uint32_t Mt = a.M.total;
+  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < Mt; m += gridDim.x * blockDim.x) {
+    int32_t ox, oc;
+    group_offsets(a.M, m, ox, oc);
+    C xv[KP];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) xv[k] = (k < (int)Kt) ? ldg(X + ox + xk[k]) : czero<C>();
+    for (uint32_t n = 0; n < Nt; ++n) {
+      C acc = czero<C>();
+      const C* yc = ys + (size_t)n * KP;
+#pragma unroll
+      for (int k = 0; k < KP; ++k) cfma(acc, xv[k], yc[k]);
+      Out[oc + cn[n]] = acc;
+    }
+  }
+}
+
+// --------------------------------------------------------------- k_generic
+template <typename C>
+__global__ __launch_bounds__(256) void k_generic(const C* __restrict__ A, const C* __restrict__ B,
+                                                 C* __restrict__ Out, GenArgs a) {
+  const uint32_t Mt = a.M.total, Nt = a.N.total, Kt = a.K.total;
+  const uint64_t total = (uint64_t)Mt * Nt;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t m = (uint32_t)(t % Mt), n = (uint32_t)(t / Mt);
+    int32_t oa, oc1, ob, oc2;
+    group_offsets(a.M, m, oa, oc1);
+    group_offsets(a.N, n, ob, oc2);
+    C acc = czero<C>();
+    for (uint32_t k = 0; k < Kt; ++k) {
+      int32_t ka, kb;
+      group_offsets(a.K, k, ka, kb);
+      cfma(acc, A[oa + ka], B[ob + kb]);
+    }
+    Out[oc1 + oc2] = acc;
+  }
+}
+
+// ---------------------------------------------------------------- k_reduce
+// Phase 1: workgroup b sums its K chunk for every output; partials[b*MN + o].
+template <typename C>
+__global__ __launch_bounds__(256) void k_reduce_partial(const C* __restrict__ A, const C* __restrict__ B,
+                                                        C* __restrict__ partials, GenArgs a) {
+  __shared__ C red[256];
+  const uint32_t Mt = a.M.total, Nt = a.N.total, Kt = a.K.total;
+  const uint32_t MN = Mt * Nt;
+  const uint32_t chunk = (Kt + gridDim.x - 1) / gridDim.x;
+  const uint32_t k0 = blockIdx.x * chunk;
+  const uint32_t k1 = min(Kt, k0 + chunk);
+  for (uint32_t o = 0; o < MN; ++o) {
+    uint32_t m = o % Mt, n = o / Mt;
+    int32_t oa, oc1, ob, oc2;
+    group_offsets(a.M, m, oa, oc1);
+    group_offsets(a.N, n, ob, oc2);
+    C acc = czero<C>();
+    for (uint32_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+      int32_t ka, kb;
+      group_offsets(a.K, k, ka, kb);
+      cfma(acc, A[oa + ka], B[ob + kb]);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) {
+        red[threadIdx.x].x += red[threadIdx.x + w].x;
+        red[threadIdx.x].y += red[threadIdx.x + w].y;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[(size_t)blockIdx.x * MN + o] = red[0];
+    __syncthreads();
+  }
+}
+
+// Phase 2: one workgroup per output sums the partials in a fixed order.
+template <typename C>
+__global__ __launch_bounds__(256) void k_reduce_final(const C* __restrict__ partials, C* __restrict__ Out,
+                                                      GenArgs a, int nblocks) {
+  __shared__ C red[256];
+  const uint32_t Mt = a.M.total, Nt = a.N.total;
+  const uint32_t MN = Mt * Nt;
+  const uint32_t o = blockIdx.x;
+  C acc = czero<C>();
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+    C p = partials[(size_t)b * MN + o];
+    acc.x += p.x; acc.y += p.y;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[threadIdx.x].x += red[threadIdx.x + w].x;
+      red[threadIdx.x].y += red[threadIdx.x + w].y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    uint32_t m = o % Mt, n = o / Mt;
+    int32_t oa, oc1, ob, oc2;
+    group_offsets(a.M, m, oa, oc1);
+    group_offsets(a.N, n, ob, oc2);
+    Out[oc1 + oc2] = red[0];
+  }
+}
+
+// ------------------------------------------------------------------ k_copy
+template <typename C>
+__global__ __launch_bounds__(256) void k_copy(const C* __restrict__ In, C* __restrict__ Out, CopyArgs a) {
+  const uint32_t T = a.G.total;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x) {
+    int32_t oi, oo;
+    group_offsets(a.G, t, oi, oo);
+    Out[oo] = In[oi];
+  }
+}
+
+inline unsigned grid_for(uint64_t work, unsigned block, unsigned cap = 8192) {
+  uint64_t g = (work + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+}  // namespace
+
+template <typename C>
+hipError_t launch_stream(const C* X, const C* Y, C* Out, const StreamArgs& a, hipStream_t s) {
+  const uint32_t Kt = a.K.total, Nt = a.N.total;
+  int KP = 1;
+  while ((uint32_t)KP < Kt) KP <<= 1;
+  size_t smem = (size_t)KP * Nt * sizeof(C) + (size_t)KP * 4 + (size_t)Nt * 4;
+  unsigned grid = grid_for(a.M.total, 256, 4096);
+  switch (KP) {
+#define PQ_CASE(P) \
+  case P: hipLaunchKernelGGL((k_stream<C, P>), dim3(grid), dim3(256), smem, s, X, Y, Out, a); break;
+    PQ_CASE(1) PQ_CASE(2) PQ_CASE(4) PQ_CASE(8) PQ_CASE(16) PQ_CASE(32)
+#undef PQ_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <typename C>
+hipError_t launch_generic(const C* A, const C* B, C* Out, const GenArgs& a, hipStream_t s) {
+  uint64_t work = (uint64_t)a.M.total * a.N.total;
+  hipLaunchKernelGGL((k_generic<C>), dim3(grid_for(work, 256)), dim3(256), 0, s, A, B, Out, a);
+  return hipGetLastError();
+}
+
+int reduce_blocks_for(uint32_t ktotal) {
+  uint32_t b = ktotal / 4096;
+  if (b < 1) b = 1;
+  if (b > 2048) b = 2048;
+  return (int)b;
+}
+
+template <typename C>
+hipError_t launch_reduce(const C* A, const C* B, C* Out, const GenArgs& a, C* partials, int nblocks,
+                         hipStream_t s) {
+  hipLaunchKernelGGL((k_reduce_partial<C>), dim3(nblocks), dim3(256), 0, s, A, B, partials, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_reduce_final<C>), dim3(a.M.total * a.N.total), dim3(256), 0, s, partials, Out, a,
+                     nblocks);
+  return hipGetLastError();
+}
+
+template <typename C>
+hipError_t launch_copy(const C* In, C* Out, const CopyArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((k_copy<C>), dim3(grid_for(a.G.total, 256)), dim3(256), 0, s, In, Out, a);
+  return hipGetLastError();
+}
+
+#define PQ_INST(C)                                                                                   \
+  template hipError_t launch_stream<C>(const C*, const C*, C*, const StreamArgs&, hipStream_t);      \
+  template hipError_t launch_generic<C>(const C*, const C*, C*, const GenArgs&, hipStream_t);        \
+  template hipError_t launch_reduce<C>(const C*, const C*, C*, const GenArgs&, C*, int, hipStream_t); \
+  template hipError_t launch_copy<C>(const C*, C*, const CopyArgs&, hipStream_t);
+PQ_INST(c128)
+PQ_INST(c64)

@@ -1,0 +1,689 @@
+// libpq_hip runtime: context (one device, one stream), a stream-ordered
+// caching allocator, the tensor table (refcounted buffers + strided views),
+// the lowering of an ncon-labelled pairwise contraction to index groups, and
+// the kernel dispatch.  The C ABI is declared in include/pq_hip.h.
+#include "pq_internal.h"
+#include "../../include/pq_hip.h"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+struct Buf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  int refs = 0;
+};
+
+struct Tensor {
+  Buf* buf = nullptr;
+  int64_t off = 0;  // element offset into buf
+  int rank = 0;
+  int64_t dims[PQ_MAX_RANK];
+  int64_t strides[PQ_MAX_RANK];
+};
+
+struct ProfEntry {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  int64_t launches = 0;
+  double bytes = 0, flops = 0;
+};
+
+int64_t numel(const Tensor& t) {
+  int64_t n = 1;
+  for (int i = 0; i < t.rank; ++i) n *= t.dims[i];
+  return n;
+}
+
+bool is_contiguous(const Tensor& t) {
+  int64_t s = 1;
+  for (int i = 0; i < t.rank; ++i) {
+    if (t.dims[i] != 1 && t.strides[i] != s) return false;
+    s *= t.dims[i];
+  }
+  return true;
+}
+
+void set_contiguous(Tensor& t) {
+  int64_t s = 1;
+  for (int i = 0; i < t.rank; ++i) {
+    t.strides[i] = s;
+    s *= t.dims[i];
+  }
+}
+
+// A mode before lowering: extent and strides in up to two tensors.
+struct Mode {
+  int64_t ext;
+  int64_t s0, s1;
+};
+
+// Drop unit extents and merge neighbours that are contiguous in both tensors.
+std::vector<Mode> merge_modes(const std::vector<Mode>& in) {
+  std::vector<Mode> out;
+  for (const Mode& m : in) {
+    if (m.ext == 1) continue;
+    if (!out.empty()) {
+      Mode& p = out.back();
+      if (m.s0 == p.s0 * p.ext && m.s1 == p.s1 * p.ext) {
+        p.ext *= m.ext;
+        continue;
+      }
+    }
+    out.push_back(m);
+  }
+  return out;
+}
+
+bool fill_group(Group& g, const std::vector<Mode>& modes) {
+  if ((int)modes.size() > PQ_MAX_MODES) return false;
+  memset(&g, 0, sizeof(g));
+  g.nmodes = (int)modes.size();
+  uint64_t total = 1;
+  for (size_t j = 0; j < modes.size(); ++j) {
+    g.ext[j] = make_fdiv((uint32_t)modes[j].ext);
+    g.s0[j] = (int32_t)modes[j].s0;
+    g.s1[j] = (int32_t)modes[j].s1;
+    total *= (uint64_t)modes[j].ext;
+  }
+  if (total >= (1ull << 31)) return false;
+  g.total = (uint32_t)total;
+  return true;
+}
+
+}  // namespace
+
+struct pq_ctx {
+  int device = 0;
+  int dtype = PQ_C128;
+  size_t esize = 16;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::unordered_map<int64_t, Tensor> tensors;
+  int64_t next_id = 1;
+  std::multimap<size_t, void*> free_blocks;
+  size_t cached = 0;
+  bool prof = false;
+  ProfEntry pe[PQ_KC_COUNT];
+  int override_kc = -1;
+  void* scratch = nullptr;  // split-K partials
+  size_t scratch_bytes = 0;
+};
+
+namespace {
+
+int fail(pq_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hipfail(pq_ctx* c, hipError_t e, const char* where) {
+  return fail(c, PQ_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+size_t round_bytes(size_t b) {
+  if (b < 256) return 256;
+  if (b < (2u << 20)) {
+    size_t p = 256;
+    while (p < b) p <<= 1;
+    return p;
+  }
+  const size_t g = 2u << 20;
+  return (b + g - 1) / g * g;
+}
+
+void* dev_alloc(pq_ctx* c, size_t bytes, size_t* got) {
+  size_t want = round_bytes(bytes);
+  auto it = c->free_blocks.lower_bound(want);
+  if (it != c->free_blocks.end() && it->first <= want + want / 4 + (2u << 20)) {
+    void* p = it->second;
+    *got = it->first;
+    c->cached -= it->first;
+    c->free_blocks.erase(it);
+    return p;
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, want) != hipSuccess) {
+    (void)hipGetLastError();
+    hipStreamSynchronize(c->stream);
+    for (auto& kv : c->free_blocks) hipFree(kv.second);
+    c->free_blocks.clear();
+    c->cached = 0;
+    if (hipMalloc(&p, want) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+  }
+  *got = want;
+  return p;
+}
+
+void dev_release(pq_ctx* c, void* p, size_t bytes) {
+  // Stream-ordered reuse: later work on the same stream runs after the
+  // kernels that still read this block.
+  c->free_blocks.emplace(bytes, p);
+  c->cached += bytes;
+}
+
+Buf* new_buf(pq_ctx* c, size_t bytes) {
+  size_t got = 0;
+  void* p = dev_alloc(c, bytes ? bytes : 1, &got);
+  if (!p) return nullptr;
+  Buf* b = new Buf;
+  b->ptr = p;
+  b->bytes = got;
+  b->refs = 1;
+  return b;
+}
+
+void unref(pq_ctx* c, Buf* b) {
+  if (--b->refs == 0) {
+    dev_release(c, b->ptr, b->bytes);
+    delete b;
+  }
+}
+
+Tensor* get(pq_ctx* c, pq_tensor h) {
+  auto it = c->tensors.find(h);
+  return it == c->tensors.end() ? nullptr : &it->second;
+}
+
+pq_tensor add(pq_ctx* c, const Tensor& t) {
+  pq_tensor id = c->next_id++;
+  c->tensors[id] = t;
+  return id;
+}
+
+struct ProfScope {
+  pq_ctx* c;
+  int kc;
+  hipEvent_t e1 = nullptr;
+  ProfScope(pq_ctx* c_, int kc_, double bytes, double flops) : c(c_), kc(kc_) {
+    if (!c->prof) return;
+    hipEvent_t e0;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, c->stream);
+    c->pe[kc].events.push_back({e0, e1});
+    c->pe[kc].launches++;
+    c->pe[kc].bytes += bytes;
+    c->pe[kc].flops += flops;
+  }
+  ~ProfScope() {
+    if (c->prof) hipEventRecord(e1, c->stream);
+  }
+};
+
+template <typename C>
+C* dptr(const Tensor& t) {
+  return reinterpret_cast<C*>(t.buf->ptr) + t.off;
+}
+
+int materialize_into(pq_ctx* c, const Tensor& src, Tensor& dst) {
+  dst = Tensor();
+  dst.rank = src.rank;
+  for (int i = 0; i < src.rank; ++i) dst.dims[i] = src.dims[i];
+  set_contiguous(dst);
+  int64_t n = numel(src);
+  dst.buf = new_buf(c, (size_t)n * c->esize);
+  if (!dst.buf) return fail(c, PQ_ERR_OOM, "device allocation failed");
+  if (n == 0) return PQ_OK;
+  std::vector<Mode> modes;
+  for (int i = 0; i < src.rank; ++i) modes.push_back({src.dims[i], src.strides[i], dst.strides[i]});
+  modes = merge_modes(modes);
+  CopyArgs a;
+  if (!fill_group(a.G, modes)) return fail(c, PQ_ERR_UNSUPPORTED, "tensor too large or too fragmented");
+  ProfScope ps(c, PQ_KC_COPY, 2.0 * n * c->esize, 0);
+  hipError_t e = (c->dtype == PQ_C128)
+                     ? launch_copy<c128>(dptr<c128>(src), dptr<c128>(dst), a, c->stream)
+                     : launch_copy<c64>(dptr<c64>(src), dptr<c64>(dst), a, c->stream);
+  if (e != hipSuccess) return hipfail(c, e, "copy kernel");
+  return PQ_OK;
+}
+
+template <typename C>
+int run_contract(pq_ctx* c, const Tensor& A, const Tensor& B, Tensor& Ct,
+                 const std::vector<Mode>& Mm, const std::vector<Mode>& Km,
+                 const std::vector<Mode>& Nm) {
+  // Mm: (ext, sA, sC), Km: (ext, sA, sB), Nm: (ext, sB, sC) -- already merged.
+  uint64_t Mt = 1, Nt = 1, Kt = 1;
+  for (auto& m : Mm) Mt *= m.ext;
+  for (auto& m : Nm) Nt *= m.ext;
+  for (auto& m : Km) Kt *= m.ext;
+  const C* pa = dptr<C>(A);
+  const C* pb = dptr<C>(B);
+  C* pc = dptr<C>(Ct);
+  const double es = (double)sizeof(C);
+  const double alg_bytes = es * (double)(Mt * Kt + Kt * Nt + Mt * Nt);
+  const double alg_flops = 8.0 * (double)Mt * (double)Nt * (double)Kt;
+  int kc = c->override_kc;
+
+  auto stream_ok_AB = [&]() {  // X = A big, Y = B small
+    size_t smem = 0;
+    uint64_t KP = 1;
+    while (KP < Kt) KP <<= 1;
+    smem = KP * Nt * sizeof(C) + KP * 4 + Nt * 4;
+    return Kt <= PQ_STREAM_KMAX && Kt * Nt <= PQ_STREAM_YMAX && smem <= 48 * 1024;
+  };
+  auto stream_ok_BA = [&]() {
+    uint64_t KP = 1;
+    while (KP < Kt) KP <<= 1;
+    size_t smem = KP * Mt * sizeof(C) + KP * 4 + Mt * 4;
+    return Kt <= PQ_STREAM_KMAX && Kt * Mt <= PQ_STREAM_YMAX && smem <= 48 * 1024;
+  };
+
+  if (kc < 0) {
+    if (stream_ok_AB() && Mt >= Nt) kc = PQ_KC_STREAM;
+    else if (stream_ok_BA()) kc = PQ_KC_STREAM;
+    else if (stream_ok_AB()) kc = PQ_KC_STREAM;
+    else if (Mt * Nt <= 16 && Kt > 64) kc = PQ_KC_REDUCE;
+    else kc = PQ_KC_GENERIC;
+  }
+
+  hipError_t e = hipSuccess;
+  if (kc == PQ_KC_STREAM) {
+    StreamArgs sa;
+    bool ab = stream_ok_AB() && (Mt >= Nt || !stream_ok_BA());
+    if (!ab && !stream_ok_BA()) return fail(c, PQ_ERR_UNSUPPORTED, "stream kernel not applicable");
+    if (ab) {
+      if (!fill_group(sa.M, Mm) || !fill_group(sa.K, Km) || !fill_group(sa.N, Nm))
+        return fail(c, PQ_ERR_UNSUPPORTED, "too many modes");
+      ProfScope ps(c, PQ_KC_STREAM, alg_bytes, alg_flops);
+      e = launch_stream<C>(pa, pb, pc, sa, c->stream);
+    } else {
+      // X = B (big), Y = A (small): K strides swap roles.
+      std::vector<Mode> K2;
+      for (auto& m : Km) K2.push_back({m.ext, m.s1, m.s0});
+      if (!fill_group(sa.M, Nm) || !fill_group(sa.K, K2) || !fill_group(sa.N, Mm))
+        return fail(c, PQ_ERR_UNSUPPORTED, "too many modes");
+      ProfScope ps(c, PQ_KC_STREAM, alg_bytes, alg_flops);
+      e = launch_stream<C>(pb, pa, pc, sa, c->stream);
+    }
+  } else if (kc == PQ_KC_REDUCE) {
+    if (Mt * Nt > 16) return fail(c, PQ_ERR_UNSUPPORTED, "reduce kernel needs M*N <= 16");
+    GenArgs ga;
+    if (!fill_group(ga.M, Mm) || !fill_group(ga.N, Nm) || !fill_group(ga.K, Km))
+      return fail(c, PQ_ERR_UNSUPPORTED, "too many modes");
+    int nb = reduce_blocks_for((uint32_t)Kt);
+    size_t need = (size_t)nb * Mt * Nt * sizeof(C);
+    if (c->scratch_bytes < need) {
+      if (c->scratch) {
+        hipStreamSynchronize(c->stream);
+        hipFree(c->scratch);
+        c->scratch = nullptr;
+        c->scratch_bytes = 0;
+      }
+      if (hipMalloc(&c->scratch, need) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, PQ_ERR_OOM, "scratch allocation failed");
+      }
+      c->scratch_bytes = need;
+    }
+    ProfScope ps(c, PQ_KC_REDUCE, alg_bytes, alg_flops);
+    e = launch_reduce<C>(pa, pb, pc, ga, reinterpret_cast<C*>(c->scratch), nb, c->stream);
+  } else if (kc == PQ_KC_GENERIC) {
+    GenArgs ga;
+    if (!fill_group(ga.M, Mm) || !fill_group(ga.N, Nm) || !fill_group(ga.K, Km))
+      return fail(c, PQ_ERR_UNSUPPORTED, "too many modes");
+    ProfScope ps(c, PQ_KC_GENERIC, alg_bytes, alg_flops);
+    e = launch_generic<C>(pa, pb, pc, ga, c->stream);
+  } else {
+    return fail(c, PQ_ERR_UNSUPPORTED, "kernel class not available for this shape");
+  }
+  if (e != hipSuccess) return hipfail(c, e, "contraction kernel");
+  return PQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pq_ctx_create(int device, int dtype, pq_ctx** out) {
+  if (!out) return PQ_ERR_ARG;
+  if (dtype != PQ_C128 && dtype != PQ_C64) return PQ_ERR_ARG;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+    (void)hipGetLastError();
+    return PQ_ERR_HIP;
+  }
+  if (hipSetDevice(device) != hipSuccess) return PQ_ERR_HIP;
+  pq_ctx* c = new pq_ctx;
+  c->device = device;
+  c->dtype = dtype;
+  c->esize = dtype == PQ_C128 ? 16 : 8;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return PQ_ERR_HIP;
+  }
+  *out = c;
+  return PQ_OK;
+}
+
+int pq_ctx_destroy(pq_ctx* c) {
+  if (!c) return PQ_ERR_ARG;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  for (auto& kv : c->tensors) unref(c, kv.second.buf);
+  c->tensors.clear();
+  for (auto& kv : c->free_blocks) hipFree(kv.second);
+  c->free_blocks.clear();
+  if (c->scratch) hipFree(c->scratch);
+  for (auto& p : c->pe)
+    for (auto& ev : p.events) {
+      hipEventDestroy(ev.first);
+      hipEventDestroy(ev.second);
+    }
+  hipStreamDestroy(c->stream);
+  delete c;
+  return PQ_OK;
+}
+
+const char* pq_last_error(const pq_ctx* c) { return c ? c->err.c_str() : "null context"; }
+int pq_ctx_dtype(const pq_ctx* c) { return c ? c->dtype : -1; }
+
+int pq_synchronize(pq_ctx* c) {
+  if (!c) return PQ_ERR_ARG;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hipfail(c, e, "hipStreamSynchronize");
+  return PQ_OK;
+}
+
+int pq_tensor_upload(pq_ctx* c, int rank, const int64_t* dims, const void* host, int host_dtype,
+                     pq_tensor* out) {
+  if (!c || !out || rank < 0 || rank > PQ_MAX_RANK || (rank > 0 && !dims))
+    return fail(c, PQ_ERR_ARG, "bad upload arguments");
+  if (host_dtype != PQ_C128 && host_dtype != PQ_C64) return fail(c, PQ_ERR_ARG, "bad host dtype");
+  Tensor t;
+  t.rank = rank;
+  for (int i = 0; i < rank; ++i) {
+    if (dims[i] < 0) return fail(c, PQ_ERR_ARG, "negative dimension");
+    t.dims[i] = dims[i];
+  }
+  set_contiguous(t);
+  int64_t n = numel(t);
+  if (n >= (1ll << 31)) return fail(c, PQ_ERR_UNSUPPORTED, "tensor has 2^31 or more elements");
+  if (n > 0 && !host) return fail(c, PQ_ERR_ARG, "null host pointer");
+  t.buf = new_buf(c, (size_t)n * c->esize);
+  if (!t.buf) return fail(c, PQ_ERR_OOM, "device allocation failed");
+  if (n > 0) {
+    std::vector<unsigned char> tmp;
+    const void* src = host;
+    if (host_dtype != c->dtype) {
+      tmp.resize((size_t)n * c->esize);
+      if (c->dtype == PQ_C64) {
+        const double* s = (const double*)host;
+        float* d = (float*)tmp.data();
+        for (int64_t i = 0; i < 2 * n; ++i) d[i] = (float)s[i];
+      } else {
+        const float* s = (const float*)host;
+        double* d = (double*)tmp.data();
+        for (int64_t i = 0; i < 2 * n; ++i) d[i] = (double)s[i];
+      }
+      src = tmp.data();
+    }
+    hipError_t e = hipMemcpyAsync(t.buf->ptr, src, (size_t)n * c->esize, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      unref(c, t.buf);
+      return hipfail(c, e, "upload");
+    }
+  }
+  *out = add(c, t);
+  return PQ_OK;
+}
+
+int pq_tensor_shape(pq_ctx* c, pq_tensor h, int* rank, int64_t* dims) {
+  if (!c || !rank) return PQ_ERR_ARG;
+  Tensor* t = get(c, h);
+  if (!t) return fail(c, PQ_ERR_NOT_FOUND, "unknown tensor handle");
+  *rank = t->rank;
+  if (dims)
+    for (int i = 0; i < t->rank; ++i) dims[i] = t->dims[i];
+  return PQ_OK;
+}
+
+int pq_tensor_download(pq_ctx* c, pq_tensor h, void* host) {
+  if (!c) return PQ_ERR_ARG;
+  Tensor* t = get(c, h);
+  if (!t) return fail(c, PQ_ERR_NOT_FOUND, "unknown tensor handle");
+  int64_t n = numel(*t);
+  if (n == 0) return PQ_OK;
+  if (!host) return fail(c, PQ_ERR_ARG, "null host pointer");
+  Tensor src = *t;
+  Tensor tmp;
+  bool own = false;
+  if (!is_contiguous(src)) {
+    int r = materialize_into(c, src, tmp);
+    if (r) return r;
+    src = tmp;
+    own = true;
+  }
+  hipError_t e = hipMemcpyAsync(host, reinterpret_cast<unsigned char*>(src.buf->ptr) + src.off * c->esize,
+                                (size_t)n * c->esize, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (own) unref(c, tmp.buf);
+  if (e != hipSuccess) return hipfail(c, e, "download");
+  return PQ_OK;
+}
+
+int pq_tensor_free(pq_ctx* c, pq_tensor h) {
+  if (!c) return PQ_ERR_ARG;
+  auto it = c->tensors.find(h);
+  if (it == c->tensors.end()) return fail(c, PQ_ERR_NOT_FOUND, "unknown tensor handle");
+  unref(c, it->second.buf);
+  c->tensors.erase(it);
+  return PQ_OK;
+}
+
+int pq_contract(pq_ctx* c, pq_tensor ha, int ra, const int32_t* ia, pq_tensor hb, int rb,
+                const int32_t* ib, pq_tensor* out) {
+  if (!c || !out) return PQ_ERR_ARG;
+  Tensor* A = get(c, ha);
+  Tensor* B = get(c, hb);
+  if (!A || !B) return fail(c, PQ_ERR_NOT_FOUND, "unknown tensor handle");
+  if (ra != A->rank || rb != B->rank) return fail(c, PQ_ERR_ARG, "label count does not match tensor rank");
+  if ((ra && !ia) || (rb && !ib)) return fail(c, PQ_ERR_ARG, "null label array");
+  // Validate labels: positive labels appear once in A and once in B.
+  for (int i = 0; i < ra; ++i) {
+    if (ia[i] == 0) return fail(c, PQ_ERR_ARG, "label 0 is not allowed");
+    for (int j = i + 1; j < ra; ++j)
+      if (ia[i] == ia[j]) return fail(c, PQ_ERR_ARG, "repeated label in A");
+  }
+  for (int i = 0; i < rb; ++i) {
+    if (ib[i] == 0) return fail(c, PQ_ERR_ARG, "label 0 is not allowed");
+    for (int j = i + 1; j < rb; ++j)
+      if (ib[i] == ib[j]) return fail(c, PQ_ERR_ARG, "repeated label in B");
+  }
+  Tensor Ct;
+  Ct.rank = 0;
+  std::vector<int> a_open, b_open;
+  std::vector<std::pair<int, int>> common;
+  for (int i = 0; i < ra; ++i) {
+    if (ia[i] < 0) {
+      for (int j = 0; j < rb; ++j)
+        if (ib[j] == ia[i]) return fail(c, PQ_ERR_ARG, "open label shared by A and B");
+      a_open.push_back(i);
+    } else {
+      int jj = -1;
+      for (int j = 0; j < rb; ++j)
+        if (ib[j] == ia[i]) jj = j;
+      if (jj < 0) return fail(c, PQ_ERR_ARG, "summed label missing from B");
+      if (A->dims[i] != B->dims[jj]) return fail(c, PQ_ERR_ARG, "summed dimensions differ");
+      common.push_back({i, jj});
+    }
+  }
+  for (int j = 0; j < rb; ++j) {
+    if (ib[j] < 0) b_open.push_back(j);
+    else {
+      bool found = false;
+      for (int i = 0; i < ra; ++i)
+        if (ia[i] == ib[j]) found = true;
+      if (!found) return fail(c, PQ_ERR_ARG, "summed label missing from A");
+    }
+  }
+  int rc = (int)(a_open.size() + b_open.size());
+  if (rc > PQ_MAX_RANK) return fail(c, PQ_ERR_UNSUPPORTED, "result rank too large");
+  Ct.rank = rc;
+  int p = 0;
+  for (int i : a_open) Ct.dims[p++] = A->dims[i];
+  for (int j : b_open) Ct.dims[p++] = B->dims[j];
+  set_contiguous(Ct);
+  int64_t nc = numel(Ct);
+  if (nc >= (1ll << 31)) return fail(c, PQ_ERR_UNSUPPORTED, "result has 2^31 or more elements");
+  Ct.buf = new_buf(c, (size_t)nc * c->esize);
+  if (!Ct.buf) return fail(c, PQ_ERR_OOM, "device allocation failed");
+  int64_t kt = 1;
+  for (auto& pr : common) kt *= A->dims[pr.first];
+  if (nc > 0 && kt == 0) {
+    hipError_t e = hipMemsetAsync(Ct.buf->ptr, 0, (size_t)nc * c->esize, c->stream);
+    if (e != hipSuccess) {
+      unref(c, Ct.buf);
+      return hipfail(c, e, "memset");
+    }
+  } else if (nc > 0) {
+    std::vector<Mode> Mm, Km, Nm;
+    p = 0;
+    for (int i : a_open) {
+      Mm.push_back({A->dims[i], A->strides[i], Ct.strides[p]});
+      ++p;
+    }
+    for (int j : b_open) {
+      Nm.push_back({B->dims[j], B->strides[j], Ct.strides[p]});
+      ++p;
+    }
+    for (auto& pr : common) Km.push_back({A->dims[pr.first], A->strides[pr.first], B->strides[pr.second]});
+    Mm = merge_modes(Mm);
+    Nm = merge_modes(Nm);
+    Km = merge_modes(Km);
+    int r = (c->dtype == PQ_C128) ? run_contract<c128>(c, *A, *B, Ct, Mm, Km, Nm)
+                                  : run_contract<c64>(c, *A, *B, Ct, Mm, Km, Nm);
+    if (r) {
+      unref(c, Ct.buf);
+      return r;
+    }
+  }
+  *out = add(c, Ct);
+  return PQ_OK;
+}
+
+int pq_permute(pq_ctx* c, pq_tensor h, const int32_t* perm, pq_tensor* out) {
+  if (!c || !out) return PQ_ERR_ARG;
+  Tensor* t = get(c, h);
+  if (!t) return fail(c, PQ_ERR_NOT_FOUND, "unknown tensor handle");
+  if (t->rank && !perm) return fail(c, PQ_ERR_ARG, "null permutation");
+  std::vector<int> seen(t->rank, 0);
+  Tensor r = *t;
+  for (int i = 0; i < t->rank; ++i) {
+    int a = perm[i] - 1;
+    if (a < 0 || a >= t->rank || seen[a]) return fail(c, PQ_ERR_ARG, "not a permutation");
+    seen[a] = 1;
+    r.dims[i] = t->dims[a];
+    r.strides[i] = t->strides[a];
+  }
+  r.buf->refs++;
+  *out = add(c, r);
+  return PQ_OK;
+}
+
+int pq_reshape(pq_ctx* c, pq_tensor h, int rank, const int64_t* dims) {
+  if (!c || rank < 0 || rank > PQ_MAX_RANK || (rank && !dims)) return fail(c, PQ_ERR_ARG, "bad reshape");
+  Tensor* t = get(c, h);
+  if (!t) return fail(c, PQ_ERR_NOT_FOUND, "unknown tensor handle");
+  int64_t n = 1;
+  for (int i = 0; i < rank; ++i) n *= dims[i];
+  if (n != numel(*t)) return fail(c, PQ_ERR_ARG, "reshape changes the number of elements");
+  if (!is_contiguous(*t)) {
+    Tensor m;
+    int r = materialize_into(c, *t, m);
+    if (r) return r;
+    unref(c, t->buf);
+    *t = m;
+  }
+  t->rank = rank;
+  for (int i = 0; i < rank; ++i) t->dims[i] = dims[i];
+  set_contiguous(*t);
+  return PQ_OK;
+}
+
+int pq_view(pq_ctx* c, pq_tensor h, int axis, int64_t lo, int64_t hi, pq_tensor* out) {
+  if (!c || !out) return PQ_ERR_ARG;
+  Tensor* t = get(c, h);
+  if (!t) return fail(c, PQ_ERR_NOT_FOUND, "unknown tensor handle");
+  if (axis < 1 || axis > t->rank) return fail(c, PQ_ERR_ARG, "axis out of range");
+  if (lo < 1 || hi > t->dims[axis - 1] || hi < lo - 1) return fail(c, PQ_ERR_ARG, "range out of bounds");
+  Tensor r = *t;
+  r.off += (lo - 1) * t->strides[axis - 1];
+  r.dims[axis - 1] = hi - lo + 1;
+  r.buf->refs++;
+  *out = add(c, r);
+  return PQ_OK;
+}
+
+int pq_materialize(pq_ctx* c, pq_tensor h, pq_tensor* out) {
+  if (!c || !out) return PQ_ERR_ARG;
+  Tensor* t = get(c, h);
+  if (!t) return fail(c, PQ_ERR_NOT_FOUND, "unknown tensor handle");
+  Tensor m;
+  int r = materialize_into(c, *t, m);
+  if (r) return r;
+  *out = add(c, m);
+  return PQ_OK;
+}
+
+int pq_tensor_device_ptr(pq_ctx* c, pq_tensor h, void** base, int64_t* off, int64_t* strides) {
+  if (!c) return PQ_ERR_ARG;
+  Tensor* t = get(c, h);
+  if (!t) return fail(c, PQ_ERR_NOT_FOUND, "unknown tensor handle");
+  if (base) *base = t->buf->ptr;
+  if (off) *off = t->off;
+  if (strides)
+    for (int i = 0; i < t->rank; ++i) strides[i] = t->strides[i];
+  return PQ_OK;
+}
+
+int pq_profile_enable(pq_ctx* c, int enable) {
+  if (!c) return PQ_ERR_ARG;
+  hipStreamSynchronize(c->stream);
+  if (enable) {
+    for (auto& p : c->pe) {
+      for (auto& ev : p.events) {
+        hipEventDestroy(ev.first);
+        hipEventDestroy(ev.second);
+      }
+      p.events.clear();
+      p.launches = 0;
+      p.bytes = p.flops = 0;
+    }
+  }
+  c->prof = enable != 0;
+  return PQ_OK;
+}
+
+int pq_profile_read(pq_ctx* c, int kc, int64_t* launches, double* total_ms, double* bytes, double* flops) {
+  if (!c || kc < 0 || kc >= PQ_KC_COUNT) return PQ_ERR_ARG;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hipfail(c, e, "profile sync");
+  double ms = 0;
+  for (auto& ev : c->pe[kc].events) {
+    float x = 0;
+    if (hipEventElapsedTime(&x, ev.first, ev.second) == hipSuccess) ms += x;
+  }
+  if (launches) *launches = c->pe[kc].launches;
+  if (total_ms) *total_ms = ms;
+  if (bytes) *bytes = c->pe[kc].bytes;
+  if (flops) *flops = c->pe[kc].flops;
+  return PQ_OK;
+}
+
+int pq_set_kernel_override(pq_ctx* c, int kc) {
+  if (!c || kc < -1 || kc >= PQ_KC_COUNT) return PQ_ERR_ARG;
+  c->override_kc = kc;
+  return PQ_OK;
+}
+
+}  // extern "C"

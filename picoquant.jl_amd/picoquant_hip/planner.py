@@ -1,0 +1,257 @@
+"""Contraction planners producing the reference's pair plans [[A, B], ...]
+(consumed by contract_network!, src/layer2.jl:294-321).
+
+  bgreedy        -- Boltzmann-greedy with the Gray-Kourtis cost |C| - alpha(|A|+|B|)
+                    (src/layer2/bgreedy_contraction.jl:17-131).  tau = 0 is the
+                    deterministic limit (argmin, first pair in scan order on ties);
+                    tau > 0 samples through the Gumbel-max form of the same
+                    Boltzmann weights, which cannot overflow (the reference's
+                    exp(-E/tau) does at 2^30-element tensors).
+  sweep_plan     -- world lines first (each qubit's tensors contracted along
+                    time into one site tensor), then the sites absorbed one by
+                    one into a growing boundary in a given site order.
+  plan_ledger    -- exact per-step shapes, MACs and bytes of a plan.
+"""
+from __future__ import annotations
+
+import heapq
+import math
+from collections import OrderedDict, defaultdict
+
+import numpy as np
+
+from .contraction import sort_indices
+
+
+def _pair_cost(A_idx, A_dims, B_idx, B_dims, alpha):
+    bset = set(B_idx)
+    aset = set(A_idx)
+    sc = 1
+    for x, d in zip(A_idx, A_dims):
+        if x not in bset:
+            sc *= d
+    for x, d in zip(B_idx, B_dims):
+        if x not in aset:
+            sc *= d
+    sa = math.prod(A_dims)
+    sb = math.prod(B_dims)
+    return sc - alpha * (sa + sb), sa, sb, sc
+
+
+def bgreedy_single(network, alpha=1.0, tau=0.0, rng=None, exact_limit=160):
+    """One bgreedy plan (src/layer2/bgreedy_contraction.jl:17-45).  Returns
+    (plan, time_cost, space_cost) with time_cost = sum sqrt(|A||B||C|)."""
+    tensors = OrderedDict((k, (list(v.indices), list(v.dims))) for k, v in network.nodes.items())
+    counter = network.counters["node"]
+    if len(tensors) <= exact_limit:
+        return _bgreedy_scan(tensors, counter, alpha, tau, rng)
+    return _bgreedy_heap(tensors, counter, alpha, tau, rng)
+
+
+def _gumbel(rng):
+    return -math.log(-math.log(max(rng.random(), 1e-300)))
+
+
+def _merge(A, B):
+    aset, bset = set(A[0]), set(B[0])
+    idx = [x for x in A[0] if x not in bset] + [x for x in B[0] if x not in aset]
+    d = dict(zip(A[0] + B[0], A[1] + B[1]))
+    return idx, [d[x] for x in idx]
+
+
+def _bgreedy_scan(tensors, counter, alpha, tau, rng):
+    plan, tcost, scost = [], 0.0, 0
+    while len(tensors) > 1:
+        items = list(tensors.items())
+        best = None
+        for i in range(len(items)):
+            ai, ad = items[i][1]
+            for j in range(i + 1, len(items)):
+                bi, bd = items[j][1]
+                e, sa, sb, sc = _pair_cost(ai, ad, bi, bd, alpha)
+                key = e if tau <= 0 else e / tau - _gumbel(rng)
+                if best is None or key < best[0]:
+                    best = (key, items[i][0], items[j][0], sa, sb, sc)
+        _, a, b, sa, sb, sc = best
+        counter += 1
+        c = f"node_{counter}"
+        C = _merge(tensors.pop(a), tensors.pop(b))
+        tensors[c] = C
+        plan.append([a, b])
+        tcost += math.sqrt(sa * sb * sc)
+        scost = max(scost, sc)
+    return plan, tcost, scost
+
+
+def _bgreedy_heap(tensors, counter, alpha, tau, rng):
+    """Same selection restricted to pairs that share an index (disconnected
+    pieces are joined by outer products at the end)."""
+    order = {k: i for i, k in enumerate(tensors)}
+    ix2t = defaultdict(set)
+    for k, (idx, _) in tensors.items():
+        for x in idx:
+            ix2t[x].add(k)
+    heap = []
+
+    def push(a, b):
+        if order[a] > order[b]:
+            a, b = b, a
+        e, _, _, _ = _pair_cost(*tensors[a], *tensors[b], alpha)
+        key = e if tau <= 0 else e / tau - _gumbel(rng)
+        heapq.heappush(heap, (key, order[a], order[b], a, b))
+
+    for x, s in ix2t.items():
+        if len(s) == 2:
+            push(*sorted(s, key=lambda k: order[k]))
+    plan, tcost, scost = [], 0.0, 0
+    nxt = len(order)
+    while len(tensors) > 1:
+        if heap:
+            _, _, _, a, b = heapq.heappop(heap)
+            if a not in tensors or b not in tensors:
+                continue
+        else:
+            it = iter(tensors)
+            a, b = next(it), next(it)
+        A, B = tensors.pop(a), tensors.pop(b)
+        sa, sb = math.prod(A[1]), math.prod(B[1])
+        C = _merge(A, B)
+        sc = math.prod(C[1])
+        counter += 1
+        c = f"node_{counter}"
+        for x in A[0]:
+            ix2t[x].discard(a)
+        for x in B[0]:
+            ix2t[x].discard(b)
+        tensors[c] = C
+        order[c] = nxt
+        nxt += 1
+        nb = set()
+        for x in C[0]:
+            ix2t[x].add(c)
+            nb |= ix2t[x]
+        nb.discard(c)
+        for o in nb:
+            push(o, c)
+        plan.append([a, b])
+        tcost += math.sqrt(sa * sb * sc)
+        scost = max(scost, sc)
+    return plan, tcost, scost
+
+
+def bgreedy(network, alpha=1.0, tau=1.0, N=10, rng=None):
+    """src/layer2/bgreedy_contraction.jl:53-66: best of N samples by time cost."""
+    rng = rng if rng is not None else np.random.default_rng()
+    best = None
+    for _ in range(N):
+        plan, t, _ = bgreedy_single(network, alpha, tau, rng)
+        if best is None or t < best[1]:
+            best = (plan, t)
+    return best
+
+
+# --------------------------------------------------------------------------- #
+# World-line / sweep planner
+# --------------------------------------------------------------------------- #
+
+
+def world_lines(network):
+    """Each qubit's nodes in time order (input, gates, output), following the
+    qubit's edge chain.  Requires every node to sit on a single qubit line
+    (two-qubit gates added with decompose=True)."""
+    lines = {}
+    owner = {}
+    for q in range(1, network.number_qubits + 1):
+        e_lab = network.input_qubits[q - 1]
+        seq = []
+        e = network.edges[e_lab]
+        if e.src is not None:
+            seq.append(e.src)
+        while True:
+            dst = network.edges[e_lab].dst
+            if dst is None:
+                break
+            seq.append(dst)
+            nxt = None
+            for x in network.nodes[dst].indices:
+                ex = network.edges[x]
+                if ex.qubit == q and ex.src == dst and not ex.virtual:
+                    nxt = x
+                    break
+            if nxt is None:
+                break
+            e_lab = nxt
+        for k in seq:
+            if k in owner:
+                raise ValueError(f"node {k} sits on qubits {owner[k]} and {q}: use decompose=True")
+            owner[k] = q
+        lines[q] = seq
+    missing = set(network.nodes) - set(owner)
+    if missing:
+        raise ValueError(f"nodes not on any qubit line: {sorted(missing)[:5]}")
+    return lines
+
+
+def sweep_plan(network, site_order):
+    """Pair plan: contract each world line into a site tensor (in time order),
+    then absorb the sites into one boundary tensor in `site_order` (1-based
+    qubits)."""
+    lines = world_lines(network)
+    counter = network.counters["node"]
+    plan = []
+    site = {}
+    for q in sorted(lines):
+        seq = lines[q]
+        cur = seq[0]
+        for k in seq[1:]:
+            plan.append([cur, k])
+            counter += 1
+            cur = f"node_{counter}"
+        site[q] = cur
+    cur = site[site_order[0]]
+    for q in site_order[1:]:
+        plan.append([cur, site[q]])
+        counter += 1
+        cur = f"node_{counter}"
+    return plan
+
+
+def grid_site_order(rows, cols, axis="col", open_last=()):
+    """Snake order over grid sites (1-based qubits, q = r*cols + c + 1); sites
+    listed in open_last go to the end (keeps open output legs out of the
+    boundary until the last steps)."""
+    seq = []
+    if axis == "col":
+        for c in range(cols):
+            rs = range(rows) if c % 2 == 0 else range(rows - 1, -1, -1)
+            seq += [r * cols + c + 1 for r in rs]
+    else:
+        for r in range(rows):
+            cs = range(cols) if r % 2 == 0 else range(cols - 1, -1, -1)
+            seq += [r * cols + c + 1 for c in cs]
+    ol = [q for q in seq if q in set(open_last)]
+    return [q for q in seq if q not in set(open_last)] + ol
+
+
+# --------------------------------------------------------------------------- #
+# Ledger
+# --------------------------------------------------------------------------- #
+
+
+def plan_ledger(network, plan):
+    """Per-step (|A|, |B|, |C|, K) for a pair plan, without touching data,
+    plus totals: MACs (the reference's Metrics.flops), max |C|."""
+    nodes = {k: (list(v.indices), list(v.dims)) for k, v in network.nodes.items()}
+    counter = network.counters["node"]
+    steps = []
+    for a, b in plan:
+        A, B = nodes.pop(a), nodes.pop(b)
+        aset, bset = set(A[0]), set(B[0])
+        dA = dict(zip(A[0], A[1]))
+        K = math.prod(d for x, d in dA.items() if x in bset)
+        C = _merge(A, B)
+        counter += 1
+        nodes[f"node_{counter}"] = C
+        steps.append((math.prod(A[1]), math.prod(B[1]), math.prod(C[1]), K))
+    macs = sum(c * k for _, _, c, k in steps)
+    return {"steps": steps, "macs": macs, "max_size": max((s[2] for s in steps), default=0)}
