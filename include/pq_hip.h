@@ -110,6 +110,10 @@ int pq_profile_read(pq_ctx* ctx, int kclass, int64_t* launches, double* total_ms
  * when the shape allows it (PQ_ERR_UNSUPPORTED otherwise). */
 int pq_set_kernel_override(pq_ctx* ctx, int kclass);
 
+/* GEMM workgroup tile override (-1 = automatic): 0:32x32 1:64x32 2:32x64
+ * 3:64x64 4:128x64 5:64x128 (complex128 uses at most 64x64). */
+int pq_set_gemm_tile(pq_ctx* ctx, int tile);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,212 @@
+// Complex GEMM on CDNA4 matrix cores with the index permutation fused into
+// operand loads and result stores (TTGT without the transposes).
+//
+//   C[m, n] = sum_k A[m, k] * B[k, n]      (per batch entry z)
+//
+// m, n, k are linear indices of the M / N / K index groups; element offsets
+// come from per-workgroup offset tables in LDS (row tables for the tile,
+// full K tables), so any stride pattern is read and written in place.
+//
+// Complex product as four real MFMAs on split planes:
+//   Cr += Ar*Br + (-Ai)*Bi ;  Ci += Ar*Bi + Ai*Br
+// complex64  -> v_mfma_f32_16x16x4_f32 (fp32 accumulate)
+// complex128 -> v_mfma_f64_16x16x4_f64 (fp64 accumulate)
+//
+// Tile: 256 lanes = 4 waves in a 2x2 arrangement; workgroup tile BM x BN,
+// each wave (BM/2) x (BN/2) as a grid of 16x16 MFMA tiles; BK = 16 per LDS
+// stage.  Lane layout of a 16x16x4 MFMA (64 lanes): A operand lane l holds
+// A[l%16][l/16], B operand B[l/16][l%16]; accumulator register r holds
+// D[4*(l/16)+r][l%16] for f32 and D[l/16+4r][l%16] for f64.
+#include "pq_internal.h"
+
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+template <typename R> struct mfma_t;
+template <> struct mfma_t<float> {
+  typedef floatx4 acc;
+  // accumulator register r of lane-group g (= lane/16) holds row 4*g + r
+  static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
+  static __device__ __forceinline__ acc op(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct mfma_t<double> {
+  typedef doublex4 acc;
+  // the f64 variant interleaves: register r of lane-group g holds row g + 4*r
+  static __device__ __forceinline__ int row(int g, int r) { return g + 4 * r; }
+  static __device__ __forceinline__ acc op(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int BK = 16;
+constexpr int KTAB_MAX = 2048;   // K tables held in LDS
+
+template <typename C, int BM, int BN, bool AK, bool BK_FAST>
+__global__ __launch_bounds__(256) void k_gemm(const C* __restrict__ A, const C* __restrict__ B,
+                                              C* __restrict__ Out, GemmArgs g) {
+  typedef typename real_of<C>::type R;
+  typedef mfma_t<R> MF;
+  constexpr int WM = BM / 2, WN = BN / 2;        // wave tile
+  constexpr int TM = WM / 16, TN = WN / 16;      // 16x16 tiles per wave
+
+  __shared__ R As_r[BK][BM + 4], As_i[BK][BM + 4];
+  __shared__ R Bs_r[BK][BN + 4], Bs_i[BK][BN + 4];
+  __shared__ int32_t offAm[BM], offCm[BM], offBn[BN], offCn[BN];
+  __shared__ int32_t offAk[KTAB_MAX], offBk[KTAB_MAX];
+
+  const uint32_t Mt = g.M.total, Nt = g.N.total, Kt = g.K.total;
+  const uint32_t m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int tid = threadIdx.x;
+  const int64_t z = blockIdx.z;
+  A += z * g.bsA;
+  B += z * g.bsB;
+  Out += z * g.bsC;
+
+  for (int t = tid; t < BM; t += 256) {
+    int32_t oa = 0, oc = 0;
+    if (m0 + t < Mt) group_offsets(g.M, m0 + t, oa, oc);
+    offAm[t] = oa;
+    offCm[t] = oc;
+  }
+  for (int t = tid; t < BN; t += 256) {
+    int32_t ob = 0, oc = 0;
+    if (n0 + t < Nt) group_offsets(g.N, n0 + t, ob, oc);
+    offBn[t] = ob;
+    offCn[t] = oc;
+  }
+  for (uint32_t t = tid; t < Kt; t += 256) {
+    int32_t oa, ob;
+    group_offsets(g.K, t, oa, ob);
+    offAk[t] = oa;
+    offBk[t] = ob;
+  }
+  __syncthreads();
+
+  const int wave = tid / 64, lane = tid % 64;
+  const int wm = (wave / 2) * WM, wn = (wave % 2) * WN;
+  const int li = lane % 16, lk = lane / 16;
+
+  typename MF::acc accr[TM][TN], acci[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        accr[i][j][r] = 0;
+        acci[i][j][r] = 0;
+      }
+
+  for (uint32_t k0 = 0; k0 < Kt; k0 += BK) {
+    // A tile BM x BK -> As[k][m]
+    for (int t = tid; t < BM * BK; t += 256) {
+      int m, k;
+      if (AK) { k = t % BK; m = t / BK; } else { m = t % BM; k = t / BM; }
+      C v;
+      v.x = 0; v.y = 0;
+      if (m0 + m < Mt && k0 + k < Kt) v = A[offAm[m] + offAk[k0 + k]];
+      As_r[k][m] = v.x;
+      As_i[k][m] = v.y;
+    }
+    // B tile BK x BN -> Bs[k][n]
+    for (int t = tid; t < BK * BN; t += 256) {
+      int n, k;
+      if (BK_FAST) { k = t % BK; n = t / BK; } else { n = t % BN; k = t / BN; }
+      C v;
+      v.x = 0; v.y = 0;
+      if (n0 + n < Nt && k0 + k < Kt) v = B[offBk[k0 + k] + offBn[n]];
+      Bs_r[k][n] = v.x;
+      Bs_i[k][n] = v.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      R ar[TM], ai[TM], br[TN], bi[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        ar[i] = As_r[kk + lk][wm + 16 * i + li];
+        ai[i] = As_i[kk + lk][wm + 16 * i + li];
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        br[j] = Bs_r[kk + lk][wn + 16 * j + li];
+        bi[j] = Bs_i[kk + lk][wn + 16 * j + li];
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          accr[i][j] = MF::op(ar[i], br[j], accr[i][j]);
+          accr[i][j] = MF::op(-ai[i], bi[j], accr[i][j]);
+          acci[i][j] = MF::op(ar[i], bi[j], acci[i][j]);
+          acci[i][j] = MF::op(ai[i], br[j], acci[i][j]);
+        }
+    }
+    __syncthreads();
+  }
+
+  // Epilogue: accumulator register r of tile (i, j) holds D[MF::row(lk, r)][li].
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int m = wm + 16 * i + MF::row(lk, r);
+        int n = wn + 16 * j + li;
+        if (m0 + m < Mt && n0 + n < Nt) {
+          C v;
+          v.x = accr[i][j][r];
+          v.y = acci[i][j][r];
+          Out[offCm[m] + offCn[n]] = v;
+        }
+      }
+}
+
+// Static LDS stays within 64 KiB: complex128 tiles up to 64x64.
+template <typename C, int BM, int BN>
+constexpr bool tile_fits() {
+  return sizeof(C) == 8 || BM * BN <= 64 * 64;
+}
+
+template <typename C, int BM, int BN>
+hipError_t launch_tile(const C* A, const C* B, C* Out, const GemmArgs& g, hipStream_t s) {
+  if constexpr (!tile_fits<C, BM, BN>()) {
+    return launch_tile<C, 64, 64>(A, B, Out, g, s);
+  }
+  dim3 grid((g.M.total + BM - 1) / BM, (g.N.total + BN - 1) / BN, g.batch);
+  if (g.a_kfast && g.b_kfast)
+    hipLaunchKernelGGL((k_gemm<C, BM, BN, true, true>), grid, dim3(256), 0, s, A, B, Out, g);
+  else if (g.a_kfast)
+    hipLaunchKernelGGL((k_gemm<C, BM, BN, true, false>), grid, dim3(256), 0, s, A, B, Out, g);
+  else if (g.b_kfast)
+    hipLaunchKernelGGL((k_gemm<C, BM, BN, false, true>), grid, dim3(256), 0, s, A, B, Out, g);
+  else
+    hipLaunchKernelGGL((k_gemm<C, BM, BN, false, false>), grid, dim3(256), 0, s, A, B, Out, g);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int gemm_k_limit() { return KTAB_MAX; }
+
+template <typename C>
+hipError_t launch_gemm(const C* A, const C* B, C* Out, const GemmArgs& g, hipStream_t s) {
+  if (g.K.total > (uint32_t)KTAB_MAX || g.batch < 1 || g.batch > 65535) return hipErrorInvalidValue;
+  switch (g.tile) {
+    case 0: return launch_tile<C, 32, 32>(A, B, Out, g, s);
+    case 1: return launch_tile<C, 64, 32>(A, B, Out, g, s);
+    case 2: return launch_tile<C, 32, 64>(A, B, Out, g, s);
+    case 3: return launch_tile<C, 64, 64>(A, B, Out, g, s);
+    case 4: return launch_tile<C, 128, 64>(A, B, Out, g, s);
+    case 5: return launch_tile<C, 64, 128>(A, B, Out, g, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template hipError_t launch_gemm<c128>(const c128*, const c128*, c128*, const GemmArgs&, hipStream_t);
+template hipError_t launch_gemm<c64>(const c64*, const c64*, c64*, const GemmArgs&, hipStream_t);

@@ -32,6 +32,11 @@ struct CopyArgs {
 // small tables instead of per-element decoding.
 struct GemmArgs {
   Group M, N, K;      // as GenArgs
+  int32_t batch;      // batch entries (grid z); entry z reads A + z*bsA ...
+  int64_t bsA, bsB, bsC;
+  int32_t a_kfast;    // A's unit-stride mode is in K: lanes walk k when loading
+  int32_t b_kfast;    // B's unit-stride mode is in K
+  int32_t tile;       // 0:32x32 1:64x32 2:32x64 3:64x64 4:128x64 5:64x128
 };
 
 template <typename C>
@@ -44,8 +49,8 @@ hipError_t launch_reduce(const C* A, const C* B, C* Out, const GenArgs& a, C* pa
 template <typename C>
 hipError_t launch_copy(const C* In, C* Out, const CopyArgs& a, hipStream_t s);
 template <typename C>
-hipError_t launch_gemm(const C* A, const C* B, C* Out, const GemmArgs& a, int32_t* tables,
-                       hipStream_t s);
+hipError_t launch_gemm(const C* A, const C* B, C* Out, const GemmArgs& a, hipStream_t s);
+int gemm_k_limit();
 
 // Number of partial blocks the split-K reduction uses for a given K extent.
 int reduce_blocks_for(uint32_t ktotal);

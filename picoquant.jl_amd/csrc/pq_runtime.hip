@@ -111,6 +111,7 @@ struct pq_ctx {
   bool prof = false;
   ProfEntry pe[PQ_KC_COUNT];
   int override_kc = -1;
+  int gemm_tile = -1;
   void* scratch = nullptr;  // split-K partials
   size_t scratch_bytes = 0;
 };
@@ -246,6 +247,17 @@ int materialize_into(pq_ctx* c, const Tensor& src, Tensor& dst) {
   return PQ_OK;
 }
 
+// Default workgroup tile for the MFMA GEMM: big tiles when both free extents
+// fill them, narrow ones otherwise (complex128 is limited to 64x64).
+int pick_tile(uint64_t Mt, uint64_t Nt, size_t esize) {
+  if (Mt <= 32 && Nt <= 32) return 0;
+  if (Nt <= 32) return 1;
+  if (Mt <= 32) return 2;
+  if (esize == 8 && Mt >= 4096 && Nt >= 128) return 5;
+  if (esize == 8 && Mt >= 4096 && Nt >= 64) return 4;
+  return 3;
+}
+
 template <typename C>
 int run_contract(pq_ctx* c, const Tensor& A, const Tensor& B, Tensor& Ct,
                  const std::vector<Mode>& Mm, const std::vector<Mode>& Km,
@@ -277,11 +289,12 @@ int run_contract(pq_ctx* c, const Tensor& A, const Tensor& B, Tensor& Ct,
     return Kt <= PQ_STREAM_KMAX && Kt * Mt <= PQ_STREAM_YMAX && smem <= 48 * 1024;
   };
 
+  const bool gemm_ok = Kt <= (uint64_t)gemm_k_limit() && Mt >= 16 && Nt >= 16 && Kt >= 4;
   if (kc < 0) {
-    if (stream_ok_AB() && Mt >= Nt) kc = PQ_KC_STREAM;
-    else if (stream_ok_BA()) kc = PQ_KC_STREAM;
-    else if (stream_ok_AB()) kc = PQ_KC_STREAM;
+    if ((stream_ok_AB() && Kt * Nt <= 64) || (stream_ok_BA() && Kt * Mt <= 64)) kc = PQ_KC_STREAM;
     else if (Mt * Nt <= 16 && Kt > 64) kc = PQ_KC_REDUCE;
+    else if (gemm_ok) kc = PQ_KC_GEMM;
+    else if (stream_ok_AB() || stream_ok_BA()) kc = PQ_KC_STREAM;
     else kc = PQ_KC_GENERIC;
   }
 
@@ -326,6 +339,20 @@ int run_contract(pq_ctx* c, const Tensor& A, const Tensor& B, Tensor& Ct,
     }
     ProfScope ps(c, PQ_KC_REDUCE, alg_bytes, alg_flops);
     e = launch_reduce<C>(pa, pb, pc, ga, reinterpret_cast<C*>(c->scratch), nb, c->stream);
+  } else if (kc == PQ_KC_GEMM) {
+    if (!gemm_ok) return fail(c, PQ_ERR_UNSUPPORTED, "gemm kernel not applicable");
+    GemmArgs ga;
+    memset(&ga, 0, sizeof(ga));
+    if (!fill_group(ga.M, Mm) || !fill_group(ga.N, Nm) || !fill_group(ga.K, Km))
+      return fail(c, PQ_ERR_UNSUPPORTED, "too many modes");
+    ga.batch = 1;
+    for (auto& m : Km) {
+      if (m.s0 == 1) ga.a_kfast = 1;
+      if (m.s1 == 1) ga.b_kfast = 1;
+    }
+    ga.tile = c->gemm_tile >= 0 ? c->gemm_tile : pick_tile(Mt, Nt, sizeof(C));
+    ProfScope ps(c, PQ_KC_GEMM, alg_bytes, alg_flops);
+    e = launch_gemm<C>(pa, pb, pc, ga, c->stream);
   } else if (kc == PQ_KC_GENERIC) {
     GenArgs ga;
     if (!fill_group(ga.M, Mm) || !fill_group(ga.N, Nm) || !fill_group(ga.K, Km))
@@ -677,6 +704,12 @@ int pq_profile_read(pq_ctx* c, int kc, int64_t* launches, double* total_ms, doub
   if (total_ms) *total_ms = ms;
   if (bytes) *bytes = c->pe[kc].bytes;
   if (flops) *flops = c->pe[kc].flops;
+  return PQ_OK;
+}
+
+int pq_set_gemm_tile(pq_ctx* c, int tile) {
+  if (!c || tile < -1 || tile > 5) return PQ_ERR_ARG;
+  c->gemm_tile = tile;
   return PQ_OK;
 }
 

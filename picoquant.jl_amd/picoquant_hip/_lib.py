@@ -52,6 +52,7 @@ SIGNATURES = {
     "pq_profile_read": (ct.c_int, [_P, ct.c_int, _I64P, ct.POINTER(ct.c_double),
                                    ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
     "pq_set_kernel_override": (ct.c_int, [_P, ct.c_int]),
+    "pq_set_gemm_tile": (ct.c_int, [_P, ct.c_int]),
 }
 
 _lib = None
@@ -186,6 +187,9 @@ class Context:
 
     def set_kernel_override(self, kc):
         self.check(self.lib.pq_set_kernel_override(self.ptr, int(kc)))
+
+    def set_gemm_tile(self, tile):
+        self.check(self.lib.pq_set_gemm_tile(self.ptr, int(tile)))
 
     def profile_enable(self, on=True):
         self.check(self.lib.pq_profile_enable(self.ptr, 1 if on else 0))
