@@ -1,0 +1,243 @@
+"""Benchmark: amplitudes/s on the 36-qubit (6x6 grid) depth-14 random circuit.
+
+Workload (one step = one full run):
+  * circuit: 6x6 grid, 14 cycles, seed 2026 (sqrtX/sqrtY/sqrtW + CZ patterns
+    ACBD), input |0...0>;
+  * 64 output bitstrings: the last grid column (qubits 6, 12, ..., 36) left
+    open, the other 30 bits seeded (PCG64, seed 2027);
+  * plan: the planner's pick among candidate orders (world-line + sweep
+    plans, Boltzmann-greedy variants) with 10 sliced bonds -> 1024 slices;
+  * batched path: every slice and all 64 amplitudes in one hipGraph launch,
+    complex64 on FP32 MFMA, FP64 cross-slice sum.
+Also reported: relative error against an FP64 run of the same plan, the plain
+FP64 GPU path (the reference's drivers on HIPBackend{ComplexF64}, one slice
+timed and scaled to 1024), a roofline for the dominant kernel class, and a
+CPU baseline (the numpy restatement of InteractiveBackend{ComplexF64} on a
+bounded sample).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import math
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+CPU_THREADS = max(1, min(16, os.cpu_count() or 1))
+os.environ.setdefault("OPENBLAS_NUM_THREADS", str(CPU_THREADS))
+os.environ.setdefault("OMP_NUM_THREADS", str(CPU_THREADS))
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "picoquant.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+ROWS, COLS, DEPTH, SEED = 6, 6, 14, 2026
+N_SLICED = 10
+METRIC = "amplitudes/s, 36-qubit d14 grid, batched+learned vs plain GPU at ≤1e-5 rel err"
+PEAK = {"mfma_fp32": 157.3e12, "mfma_fp64": 78.6e12, "hbm": 8.0e12}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_network(dtype, backend=None):
+    import picoquant_hip as P
+    from picoquant_hip import circuits as C
+    n = ROWS * COLS
+    gates = C.grid_circuit_gates(ROWS, COLS, DEPTH, SEED)
+    tn = P.network_from_gates(n, gates, backend=backend or P.HIPBackend(dtype), decompose=True)
+    P.add_input(tn, "0" * n)
+    bits = C.seeded_bits(n, SEED + 1)
+    open_q = [r * COLS + COLS for r in range(ROWS)]
+    cfg = ["x" if (q + 1) in open_q else str(bits[q]) for q in range(n)]
+    P.add_output(tn, "".join(cfg))
+    return tn, open_q
+
+
+def choose_plan(tn, open_q):
+    from picoquant_hip import planner as PL
+    from picoquant_hip import learned
+    return learned.select_plan(tn, ROWS, COLS, open_q, N_SLICED)
+
+
+def plain_fp64_slice_time(plan, sliced, open_q):
+    """The reference's drivers on HIPBackend{ComplexF64}: rebuild the network,
+    slice_tensor_network at slice 0, contract_network! with the same plan."""
+    import picoquant_hip as P
+    from picoquant_hip.slicing import slice_with_map, translate_plan
+    tn, _ = build_network(np.complex128)
+    old = tn.counters["node"]
+    t0 = time.perf_counter()
+    m = slice_with_map(tn, sliced, [1] * len(sliced))
+    plan2 = translate_plan(plan, m, old, tn.counters["node"])
+    P.contract_network(tn, plan2, "vector")
+    res = tn.backend.load_tensor_data("result")
+    return time.perf_counter() - t0, res
+
+
+def cpu_baseline(plan, sliced, cap_s=20.0):
+    """oracle/pq_oracle.py (numpy: permute + BLAS zgemm, InteractiveBackend's
+    algorithm) on slice 0 of the same sliced network and plan, stopped after
+    cap_s seconds; the time is scaled by the MACs done to the whole slice."""
+    from oracle import pq_oracle as O
+    from picoquant_hip import circuits as C
+    from picoquant_hip import planner as PL
+    from picoquant_hip.slicing import translate_plan
+    n = ROWS * COLS
+    tn = O.network_from_gates(n, O.grid_circuit_gates(ROWS, COLS, DEPTH, SEED), decompose=True)
+    O.add_input(tn, "0" * n)
+    bits = C.seeded_bits(n, SEED + 1)
+    open_q = [r * COLS + COLS for r in range(ROWS)]
+    O.add_output(tn, "".join("x" if (q + 1) in open_q else str(bits[q]) for q in range(n)))
+    old = tn.counters["node"]
+    label_map = {}
+    for b in sliced:
+        for nd in (tn.edges[b].src, tn.edges[b].dst):
+            view = f"node_{tn.counters['node'] + 1}"
+            O.replace_with_view(tn, nd, b, (1, 1))
+            label_map[nd] = view
+    def cur(x):
+        while x in label_map:
+            x = label_map[x]
+        return x
+    plan2 = translate_plan(plan, {k: cur(k) for k in label_map}, old, tn.counters["node"])
+    led = [c * k for _, _, c, k in PL.plan_ledger(tn, plan2)["steps"]]
+    total = sum(led)
+    done = 0
+    t0 = time.perf_counter()
+    steps = 0
+    for (a, b), macs in zip(plan2, led):
+        O.contract_pair(tn, a, b)
+        done += macs
+        steps += 1
+        if time.perf_counter() - t0 > cap_s:
+            break
+    el = time.perf_counter() - t0
+    t_slice = el * total / max(done, 1)
+    return {"t_slice_s": t_slice, "steps_done": steps, "steps": len(plan2), "mac_fraction": done / total,
+            "elapsed_s": el}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the slices would shard "
+              "across GPUs with one FP64 reduction at the end, which this build does not run.")
+        sys.exit(2)
+    import picoquant_hip as P
+    from picoquant_hip import _lib
+    from picoquant_hip.executor import compile_plan
+
+    t0 = time.perf_counter()
+    tn, open_q = build_network(np.complex64)
+    sel = choose_plan(tn, open_q)
+    plan, sliced = sel["plan"], sel["sliced"]
+    log(f"[bench] network+plan {time.perf_counter() - t0:.1f}s: {sel['summary']}")
+    p = compile_plan(tn, plan, sliced, group_bits=sel.get("group_bits"), kc=sel.get("kc"),
+                     tiles=sel.get("tiles"))
+    st = p.stats()
+    log(f"[bench] compiled: arena {st['arena_bytes'] / 2**30:.2f} GiB, {st['launches']} launches/run, "
+        f"group_bits {p.group_bits}")
+    ctx = tn.backend.ctx
+    for _ in range(args.warmup):
+        p.execute(True)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        p.execute(True)
+    ctx.synchronize()
+    dt = time.perf_counter() - t1
+    amps = 64 * args.steps / dt
+    res64 = p.result().reshape(-1)
+    log(f"[bench] timed {args.steps} runs in {dt:.3f}s -> {amps:.1f} amplitudes/s")
+
+    # Roofline of the dominant kernel class from an event-timed run.
+    ms = p.profile()
+    per = {}
+    for i in range(len(plan)):
+        inf = p.step_info(i)
+        e = per.setdefault(inf["kc"], {"ms": 0.0, "flops": 0.0, "bytes": 0.0, "steps": 0})
+        e["ms"] += ms[i]
+        e["flops"] += inf["flops"]
+        e["bytes"] += inf["bytes"]
+        e["steps"] += 1
+    dom = max(per, key=lambda k: per[k]["ms"])
+    d = per[dom]
+    if dom == _lib.KC_GEMM:
+        achieved = d["flops"] / (d["ms"] * 1e-3)
+        roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["mfma_fp32"] / 1e12,
+                "unit": "TFLOP/s", "frac": achieved / PEAK["mfma_fp32"]}
+    else:
+        achieved = d["bytes"] / (d["ms"] * 1e-3)
+        roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK["hbm"] / 1e9,
+                "unit": "GB/s", "frac": achieved / PEAK["hbm"]}
+    roof["traffic"] = None
+    roof["kernel_class"] = _lib.KC_NAMES[dom]
+    roof["class_ms_per_run"] = d["ms"]
+    roof["classes"] = {_lib.KC_NAMES[k]: round(v["ms"], 3) for k, v in per.items()}
+    p.close()
+
+    # FP64 run of the same plan: the accuracy reference for the complex64 path.
+    tn64, _ = build_network(np.complex128)
+    p64 = compile_plan(tn64, plan, sliced, group_bits=sel.get("group_bits"))
+    p64.execute(True)
+    ref = p64.result().reshape(-1)
+    ctx64 = tn64.backend.ctx
+    ctx64.synchronize()
+    t2 = time.perf_counter()
+    p64.execute(True)
+    ctx64.synchronize()
+    t_fp64_batched = time.perf_counter() - t2
+    p64.close()
+    rel_err = float(np.max(np.abs(res64 - ref) / np.abs(ref)))
+    log(f"[bench] max rel err vs FP64 {rel_err:.2e}; batched FP64 run {t_fp64_batched:.3f}s")
+
+    plain = None
+    if not args.no_plain:
+        t_slice, res_slice = plain_fp64_slice_time(plan, sliced, open_q)
+        plain = {"amps_per_s": 64 / (t_slice * 2 ** N_SLICED), "t_one_slice_s": t_slice,
+                 "sample": "slice 0 of 1024 through contract_network! on HIPBackend{ComplexF64}, x1024"}
+        log(f"[bench] plain FP64 path: one slice {t_slice:.2f}s -> {plain['amps_per_s']:.4f} amps/s")
+
+    cpu = None
+    if not args.no_cpu:
+        cb = cpu_baseline(plan, sliced)
+        cpu = {"value": 64 / (cb["t_slice_s"] * 2 ** N_SLICED), "unit": "amplitudes/s", "cores": CPU_THREADS,
+               "kind": "port",
+               "sample": (f"numpy oracle (permute + OpenBLAS zgemm, complex128) on slice 0: first "
+                          f"{cb['steps_done']}/{cb['steps']} steps = {100 * cb['mac_fraction']:.1f}% of the "
+                          f"slice's MACs in {cb['elapsed_s']:.1f}s, scaled to the slice and x1024 slices")}
+        log(f"[bench] cpu baseline {cpu['value']:.3g} amps/s")
+
+    out = {
+        "metric": METRIC, "value": amps, "unit": "amplitudes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c64 (fp32 MFMA), fp64 slice sum",
+        "data": "synthetic",
+        "config": {"workload": "6x6 grid depth-14 random circuit (seed 2026), 10 sliced bonds = 1024 slices, "
+                               "64 bitstrings (last column open), one hipGraph per run",
+                   "qubits": 36, "depth": DEPTH, "slices": 2 ** N_SLICED, "bitstrings": 64,
+                   "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "rel_err_vs_fp64": rel_err,
+        "plain_fp64_gpu": plain,
+        "speedup_vs_plain": (amps / plain["amps_per_s"]) if plain else None,
+        "batched_fp64_amps_per_s": 64 / t_fp64_batched,
+        "plan": sel["summary"],
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -114,6 +114,54 @@ int pq_set_kernel_override(pq_ctx* ctx, int kclass);
  * 3:64x64 4:128x64 5:64x128 (complex128 uses at most 64x64). */
 int pq_set_gemm_tile(pq_ctx* ctx, int tile);
 
+/* ------------------------------------------------------------------------
+ * Planned execution: a whole pair plan (contract_network!, src/layer2.jl:
+ * 294-321) over every slice of a sliced network (src/layer2/slicing.jl),
+ * summed over slices in FP64, as one stream-ordered program that
+ * pq_plan_execute can capture into a single hipGraph and replay.
+ * Slots are tensors: leaves (context tensors that keep their sliced bonds)
+ * or intermediates (in a plan-owned arena).  Each sliced bond has extent 2;
+ * slice s sets bond j to bit j of s.  Slots that depend on a sliced bond are
+ * "batched": one tensor per slice of a slice group (2^group_bits slices).
+ * Modes are (extent, stride, stride) triples in elements of one slice:
+ * M = A-open (stride in A, in C), N = B-open (in B, in C), K = summed (in A,
+ * in B).  There is no counterpart in the reference, whose slices are
+ * rebuilt and contracted one by one and summed on the host
+ * (test/layer2_tests.jl:436-450).
+ * ---------------------------------------------------------------------- */
+typedef struct pq_plan pq_plan;
+typedef struct {
+  int32_t n_slots;
+  const int64_t* slot_elems;        /* per-slice elements (leaves: all elements) */
+  const int32_t* slot_batched;      /* 1 if the slot depends on a sliced bond */
+  const pq_tensor* slot_leaf;       /* leaf tensor, 0 for intermediates */
+  int32_t n_sliced;                 /* number of sliced bonds (extent 2 each) */
+  const int64_t* leaf_slice_stride; /* [n_slots * n_sliced] stride of bond j in a leaf, 0 if absent */
+  int32_t group_bits;               /* slices per launch = 2^group_bits (bonds 0..group_bits-1) */
+  int32_t n_steps;
+  const int32_t* step_slots;        /* [n_steps * 3] A, B, C slots */
+  const int32_t* step_kc;           /* [n_steps] PQ_KC_* or -1 (may be NULL) */
+  const int32_t* step_tile;         /* [n_steps] GEMM tile or -1 (may be NULL) */
+  const int32_t* step_nmodes;       /* [n_steps * 3] mode counts of M, N, K */
+  const int64_t* step_modes;        /* concatenated (extent, s0, s1) triples */
+  int32_t out_slot;                 /* per-slice result slot */
+} pq_plan_desc;
+
+int pq_plan_create(pq_ctx* ctx, const pq_plan_desc* desc, pq_plan** out);
+/* Enqueue one full run (all slices + FP64 accumulation); use_graph=1 captures
+ * the run into a hipGraph on first use and replays it afterwards. */
+int pq_plan_execute(pq_ctx* ctx, pq_plan* plan, int use_graph);
+/* Copy the FP64 result (2 * out_elems doubles, interleaved) to the host. */
+int pq_plan_result(pq_ctx* ctx, pq_plan* plan, double* host);
+int pq_plan_stats(const pq_plan* plan, int64_t* arena_bytes, int64_t* launches_per_run,
+                  double* alg_flops, double* alg_bytes, int64_t* out_elems);
+int pq_plan_step_info(const pq_plan* plan, int step, int* kclass, int* tile, int* dependent,
+                      double* alg_flops, double* alg_bytes);
+/* One eager run with HIP events around every step; step_ms[i] = time of step i
+ * summed over all slice groups. */
+int pq_plan_profile(pq_ctx* ctx, pq_plan* plan, double* step_ms);
+int pq_plan_destroy(pq_ctx* ctx, pq_plan* plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,3 +90,29 @@ __device__ __forceinline__ void group_offsets(const Group& g, uint32_t i, int32_
     i = q;
   }
 }
+
+// Batch group: entries of a batched contraction (e.g. the slices of one slice
+// group).  Entry b reads A at A + offA(b), B at B + offB(b) and writes C at
+// C + offC(b); each mode has its own stride per tensor (0 = shared operand).
+#define PQ_MAX_BMODES 12
+struct Batch {
+  int32_t nmodes;
+  uint32_t total;
+  FDiv ext[PQ_MAX_BMODES];
+  int64_t sA[PQ_MAX_BMODES];
+  int64_t sB[PQ_MAX_BMODES];
+  int64_t sC[PQ_MAX_BMODES];
+};
+
+__device__ __forceinline__ void batch_offsets(const Batch& L, uint32_t b, int64_t& oa, int64_t& ob,
+                                              int64_t& oc) {
+  oa = 0; ob = 0; oc = 0;
+  for (int j = 0; j < L.nmodes; ++j) {
+    uint32_t q = fdiv_q(b, L.ext[j]);
+    int64_t r = (int64_t)(b - q * L.ext[j].d);
+    oa += r * L.sA[j];
+    ob += r * L.sB[j];
+    oc += r * L.sC[j];
+    b = q;
+  }
+}

@@ -61,10 +61,13 @@ __global__ __launch_bounds__(256) void k_gemm(const C* __restrict__ A, const C* 
   const uint32_t Mt = g.M.total, Nt = g.N.total, Kt = g.K.total;
   const uint32_t m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   const int tid = threadIdx.x;
-  const int64_t z = blockIdx.z;
-  A += z * g.bsA;
-  B += z * g.bsB;
-  Out += z * g.bsC;
+  {
+    int64_t oa, ob, oc;
+    batch_offsets(g.L, blockIdx.z, oa, ob, oc);
+    A += oa;
+    B += ob;
+    Out += oc;
+  }
 
   for (int t = tid; t < BM; t += 256) {
     int32_t oa = 0, oc = 0;
@@ -178,7 +181,7 @@ hipError_t launch_tile(const C* A, const C* B, C* Out, const GemmArgs& g, hipStr
   if constexpr (!tile_fits<C, BM, BN>()) {
     return launch_tile<C, 64, 64>(A, B, Out, g, s);
   }
-  dim3 grid((g.M.total + BM - 1) / BM, (g.N.total + BN - 1) / BN, g.batch);
+  dim3 grid((g.M.total + BM - 1) / BM, (g.N.total + BN - 1) / BN, g.L.total);
   if (g.a_kfast && g.b_kfast)
     hipLaunchKernelGGL((k_gemm<C, BM, BN, true, true>), grid, dim3(256), 0, s, A, B, Out, g);
   else if (g.a_kfast)
@@ -196,7 +199,7 @@ int gemm_k_limit() { return KTAB_MAX; }
 
 template <typename C>
 hipError_t launch_gemm(const C* A, const C* B, C* Out, const GemmArgs& g, hipStream_t s) {
-  if (g.K.total > (uint32_t)KTAB_MAX || g.batch < 1 || g.batch > 65535) return hipErrorInvalidValue;
+  if (g.K.total > (uint32_t)KTAB_MAX || g.L.total < 1 || g.L.total > 65535) return hipErrorInvalidValue;
   switch (g.tile) {
     case 0: return launch_tile<C, 32, 32>(A, B, Out, g, s);
     case 1: return launch_tile<C, 64, 32>(A, B, Out, g, s);

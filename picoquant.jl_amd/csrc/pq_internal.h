@@ -14,12 +14,14 @@
 
 struct StreamArgs {
   Group M, K, N;
+  Batch L;   // sA = stride in X, sB = stride in Y, sC = stride in C
 };
 
 // General contraction (any shape):
 //   M: A-open (s0 = A, s1 = C); N: B-open (s0 = B, s1 = C); K: (s0 = A, s1 = B)
 struct GenArgs {
   Group M, N, K;
+  Batch L;
 };
 
 // Strided copy: group over output linear order, s0 = input stride, s1 = output stride.
@@ -32,8 +34,7 @@ struct CopyArgs {
 // small tables instead of per-element decoding.
 struct GemmArgs {
   Group M, N, K;      // as GenArgs
-  int32_t batch;      // batch entries (grid z); entry z reads A + z*bsA ...
-  int64_t bsA, bsB, bsC;
+  Batch L;            // batch entries on grid z
   int32_t a_kfast;    // A's unit-stride mode is in K: lanes walk k when loading
   int32_t b_kfast;    // B's unit-stride mode is in K
   int32_t tile;       // 0:32x32 1:64x32 2:32x64 3:64x64 4:128x64 5:64x128

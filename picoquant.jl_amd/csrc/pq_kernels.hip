@@ -26,6 +26,13 @@ __global__ __launch_bounds__(256) void k_stream(const C* __restrict__ X, const C
                                                 C* __restrict__ Out, StreamArgs a) {
   extern __shared__ unsigned char smem_raw[];
   const uint32_t Kt = a.K.total, Nt = a.N.total;
+  {
+    int64_t ox, oy, oc;
+    batch_offsets(a.L, blockIdx.z, ox, oy, oc);
+    X += ox;
+    Y += oy;
+    Out += oc;
+  }
   C* ys = reinterpret_cast<C*>(smem_raw);                              // [KP][Nt]
   int32_t* xk = reinterpret_cast<int32_t*>(ys + (size_t)KP * Nt);      // [KP]
   int32_t* cn = xk + KP;                                               // [Nt]
@@ -74,6 +81,13 @@ __global__ __launch_bounds__(256) void k_generic(const C* __restrict__ A, const 
                                                  C* __restrict__ Out, GenArgs a) {
   const uint32_t Mt = a.M.total, Nt = a.N.total, Kt = a.K.total;
   const uint64_t total = (uint64_t)Mt * Nt;
+  {
+    int64_t oa, ob, oc;
+    batch_offsets(a.L, blockIdx.z, oa, ob, oc);
+    A += oa;
+    B += ob;
+    Out += oc;
+  }
   for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t m = (uint32_t)(t % Mt), n = (uint32_t)(t / Mt);
@@ -98,6 +112,13 @@ __global__ __launch_bounds__(256) void k_reduce_partial(const C* __restrict__ A,
   __shared__ C red[256];
   const uint32_t Mt = a.M.total, Nt = a.N.total, Kt = a.K.total;
   const uint32_t MN = Mt * Nt;
+  {
+    int64_t oa, ob, oc;
+    batch_offsets(a.L, blockIdx.z, oa, ob, oc);
+    A += oa;
+    B += ob;
+  }
+  partials += (size_t)blockIdx.z * gridDim.x * MN;
   const uint32_t chunk = (Kt + gridDim.x - 1) / gridDim.x;
   const uint32_t k0 = blockIdx.x * chunk;
   const uint32_t k1 = min(Kt, k0 + chunk);
@@ -134,6 +155,12 @@ __global__ __launch_bounds__(256) void k_reduce_final(const C* __restrict__ part
   const uint32_t Mt = a.M.total, Nt = a.N.total;
   const uint32_t MN = Mt * Nt;
   const uint32_t o = blockIdx.x;
+  {
+    int64_t oa, ob, oc;
+    batch_offsets(a.L, blockIdx.z, oa, ob, oc);
+    Out += oc;
+  }
+  partials += (size_t)blockIdx.z * nblocks * MN;
   C acc = czero<C>();
   for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
     C p = partials[(size_t)b * MN + o];
@@ -183,10 +210,10 @@ hipError_t launch_stream(const C* X, const C* Y, C* Out, const StreamArgs& a, hi
   int KP = 1;
   while ((uint32_t)KP < Kt) KP <<= 1;
   size_t smem = (size_t)KP * Nt * sizeof(C) + (size_t)KP * 4 + (size_t)Nt * 4;
-  unsigned grid = grid_for(a.M.total, 256, 4096);
+  unsigned grid = grid_for(a.M.total, 256, a.L.total > 1 ? 1024 : 4096);
   switch (KP) {
 #define PQ_CASE(P) \
-  case P: hipLaunchKernelGGL((k_stream<C, P>), dim3(grid), dim3(256), smem, s, X, Y, Out, a); break;
+  case P: hipLaunchKernelGGL((k_stream<C, P>), dim3(grid, 1, a.L.total), dim3(256), smem, s, X, Y, Out, a); break;
     PQ_CASE(1) PQ_CASE(2) PQ_CASE(4) PQ_CASE(8) PQ_CASE(16) PQ_CASE(32)
 #undef PQ_CASE
     default: return hipErrorInvalidValue;
@@ -197,7 +224,8 @@ hipError_t launch_stream(const C* X, const C* Y, C* Out, const StreamArgs& a, hi
 template <typename C>
 hipError_t launch_generic(const C* A, const C* B, C* Out, const GenArgs& a, hipStream_t s) {
   uint64_t work = (uint64_t)a.M.total * a.N.total;
-  hipLaunchKernelGGL((k_generic<C>), dim3(grid_for(work, 256)), dim3(256), 0, s, A, B, Out, a);
+  hipLaunchKernelGGL((k_generic<C>), dim3(grid_for(work, 256, a.L.total > 1 ? 1024 : 8192), 1, a.L.total),
+                     dim3(256), 0, s, A, B, Out, a);
   return hipGetLastError();
 }
 
@@ -211,11 +239,11 @@ int reduce_blocks_for(uint32_t ktotal) {
 template <typename C>
 hipError_t launch_reduce(const C* A, const C* B, C* Out, const GenArgs& a, C* partials, int nblocks,
                          hipStream_t s) {
-  hipLaunchKernelGGL((k_reduce_partial<C>), dim3(nblocks), dim3(256), 0, s, A, B, partials, a);
+  hipLaunchKernelGGL((k_reduce_partial<C>), dim3(nblocks, 1, a.L.total), dim3(256), 0, s, A, B, partials, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_reduce_final<C>), dim3(a.M.total * a.N.total), dim3(256), 0, s, partials, Out, a,
-                     nblocks);
+  hipLaunchKernelGGL((k_reduce_final<C>), dim3(a.M.total * a.N.total, 1, a.L.total), dim3(256), 0, s, partials,
+                     Out, a, nblocks);
   return hipGetLastError();
 }
 

@@ -1,0 +1,442 @@
+// Planned contraction executor: a whole contraction plan (the pair plan of
+// contract_network!, src/layer2.jl:294-321) over all slices of a sliced
+// network (src/layer2/slicing.jl), run as one stream-ordered program that
+// can be captured into a single hipGraph.
+//
+//  * Slots: every leaf and intermediate tensor.  Leaves are context tensors
+//    (holding their sliced bonds); intermediates live in one arena whose
+//    offsets come from a liveness pass at plan creation.
+//  * Slices: each sliced bond has extent 2.  Slice s fixes bond j to bit j
+//    of s.  Steps whose operands do not depend on any sliced bond run once
+//    (phase 1); the others (phase 2) run once per slice group, batched over
+//    the 2^group_bits slices of the group on grid z, reading leaves through
+//    per-bond strides and batched intermediates through entry strides.
+//  * The per-slice result is summed over slices in FP64 (cross-slice
+//    accumulation), deterministic in slice order.
+#include "pq_rt.h"
+
+#include <algorithm>
+#include <cstring>
+
+using namespace pqrt;
+
+namespace {
+
+__global__ void k_fill_zero(double* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+// out[j] += sum_b C[b * stride + j] for b < nb (fixed order).
+template <typename C>
+__global__ void k_accum(const C* __restrict__ src, int64_t elems, int64_t stride, int nb, double* __restrict__ out) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < elems; j += (int64_t)gridDim.x * blockDim.x) {
+    double re = out[2 * j], im = out[2 * j + 1];
+    for (int b = 0; b < nb; ++b) {
+      C v = src[b * stride + j];
+      re += (double)v.x;
+      im += (double)v.y;
+    }
+    out[2 * j] = re;
+    out[2 * j + 1] = im;
+  }
+}
+
+struct Slot {
+  int64_t elems = 0;      // per batch entry
+  bool batched = false;
+  pq_tensor leaf = 0;
+  std::vector<int64_t> slice_stride;  // leaves: per sliced bond
+  int64_t arena_off = -1;             // intermediates
+  int producer = -1;
+  int last_use = -1;
+};
+
+struct PStep {
+  int a, b, c;
+  bool dep;
+  StepSpec spec;           // resolved; batch strides for phase 2 steps
+  double flops = 0, bytes = 0;
+};
+
+}  // namespace
+
+struct pq_plan {
+  std::vector<Slot> slots;
+  std::vector<PStep> steps;
+  std::vector<int> order1, order2;   // step indices of phase 1 / phase 2
+  int n_sliced = 0, group_bits = 0;
+  int64_t n_groups = 1;
+  int out_slot = -1;
+  void* arena = nullptr;
+  size_t arena_bytes = 0;
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  double* out_dev = nullptr;
+  int64_t out_elems = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int64_t launches = 0;
+  double alg_flops = 0, alg_bytes = 0;
+  std::vector<hipEvent_t> ev;  // per-step event pairs for pq_plan_profile
+};
+
+namespace {
+
+void* slot_ptr(pq_ctx* c, pq_plan* p, int s, int64_t group) {
+  Slot& sl = p->slots[s];
+  if (sl.leaf) {
+    Tensor* t = get(c, sl.leaf);
+    int64_t off = t->off;
+    for (int j = p->group_bits; j < p->n_sliced; ++j)
+      if ((group >> (j - p->group_bits)) & 1) off += sl.slice_stride[j];
+    return (unsigned char*)t->buf->ptr + off * c->esize;
+  }
+  return (unsigned char*)p->arena + sl.arena_off;
+}
+
+int64_t span_of(const std::vector<Mode>& ms, bool first) {
+  int64_t s = 0;
+  for (auto& m : ms) s += (m.ext - 1) * (first ? m.s0 : m.s1);
+  return s;
+}
+
+int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
+  int64_t nb = p->out_elems;
+  unsigned g = (unsigned)std::min<int64_t>((nb + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_fill_zero, dim3(std::max(1u, g)), dim3(256), 0, c->stream, p->out_dev, 2 * nb);
+  if (hipGetLastError() != hipSuccess) return fail(c, PQ_ERR_HIP, "fill kernel");
+  auto do_step = [&](int i, int64_t group) -> int {
+    PStep& st = p->steps[i];
+    if (profile_steps) hipEventRecord(p->ev[2 * i], c->stream);
+    int r = launch_step(c, slot_ptr(c, p, st.a, group), slot_ptr(c, p, st.b, group), slot_ptr(c, p, st.c, group),
+                        st.spec, p->scratch);
+    if (profile_steps) hipEventRecord(p->ev[2 * i + 1], c->stream);
+    return r;
+  };
+  for (int i : p->order1) {
+    int r = do_step(i, 0);
+    if (r) return r;
+  }
+  const Slot& os = p->slots[p->out_slot];
+  int nbatch = os.batched ? (1 << p->group_bits) : 1;
+  for (int64_t grp = 0; grp < p->n_groups; ++grp) {
+    for (int i : p->order2) {
+      int r = do_step(i, grp);
+      if (r) return r;
+    }
+    const void* src = slot_ptr(c, p, p->out_slot, grp);
+    unsigned ga = (unsigned)std::min<int64_t>((os.elems + 255) / 256, 1024);
+    if (c->dtype == PQ_C128)
+      hipLaunchKernelGGL((k_accum<c128>), dim3(std::max(1u, ga)), dim3(256), 0, c->stream, (const c128*)src,
+                         os.elems, os.elems, nbatch, p->out_dev);
+    else
+      hipLaunchKernelGGL((k_accum<c64>), dim3(std::max(1u, ga)), dim3(256), 0, c->stream, (const c64*)src,
+                         os.elems, os.elems, nbatch, p->out_dev);
+    if (hipGetLastError() != hipSuccess) return fail(c, PQ_ERR_HIP, "accumulate kernel");
+  }
+  return PQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
+  if (!c || !d || !out) return PQ_ERR_ARG;
+  if (d->n_slots <= 0 || d->n_steps < 0 || d->n_sliced < 0 || d->n_sliced > 40 || d->group_bits < 0 ||
+      d->group_bits > d->n_sliced || d->group_bits > 16 || d->out_slot < 0 || d->out_slot >= d->n_slots)
+    return fail(c, PQ_ERR_ARG, "bad plan header");
+  pq_plan* p = new pq_plan;
+  p->n_sliced = d->n_sliced;
+  p->group_bits = d->group_bits;
+  p->n_groups = (int64_t)1 << (d->n_sliced - d->group_bits);
+  p->out_slot = d->out_slot;
+  const int G = 1 << d->group_bits;
+  auto bad = [&](const std::string& m) {
+    delete p;
+    return fail(c, PQ_ERR_ARG, m);
+  };
+  p->slots.resize(d->n_slots);
+  for (int s = 0; s < d->n_slots; ++s) {
+    Slot& sl = p->slots[s];
+    sl.elems = d->slot_elems[s];
+    sl.batched = d->slot_batched[s] != 0;
+    sl.leaf = d->slot_leaf[s];
+    if (sl.elems < 1 || sl.elems >= (1ll << 31)) return bad("slot size out of range");
+    if (sl.leaf) {
+      Tensor* t = get(c, sl.leaf);
+      if (!t) return bad("unknown leaf tensor");
+      if (sl.batched) return bad("a leaf cannot be batched");
+      int64_t span = 0;
+      for (int i = 0; i < t->rank; ++i) span += (t->dims[i] - 1) * t->strides[i];
+      if (span + 1 > (int64_t)(t->buf->bytes / c->esize) - t->off) return bad("leaf view exceeds its buffer");
+      sl.slice_stride.assign(d->n_sliced, 0);
+      for (int j = 0; j < d->n_sliced; ++j) sl.slice_stride[j] = d->leaf_slice_stride[(int64_t)s * d->n_sliced + j];
+    }
+  }
+  // Steps.
+  int64_t mpos = 0;
+  std::vector<int> produced(d->n_slots, -1);
+  p->steps.resize(d->n_steps);
+  for (int i = 0; i < d->n_steps; ++i) {
+    PStep& st = p->steps[i];
+    st.a = d->step_slots[3 * i];
+    st.b = d->step_slots[3 * i + 1];
+    st.c = d->step_slots[3 * i + 2];
+    for (int x : {st.a, st.b, st.c})
+      if (x < 0 || x >= d->n_slots) return bad("step slot out of range");
+    if (p->slots[st.c].leaf || produced[st.c] >= 0) return bad("step output must be a fresh intermediate");
+    for (int x : {st.a, st.b})
+      if (!p->slots[x].leaf && produced[x] < 0) return bad("step reads a slot before it is produced");
+    produced[st.c] = i;
+    p->slots[st.c].producer = i;
+    p->slots[st.a].last_use = i;
+    p->slots[st.b].last_use = i;
+    const int nm = d->step_nmodes[3 * i], nn = d->step_nmodes[3 * i + 1], nk = d->step_nmodes[3 * i + 2];
+    auto take = [&](int n) {
+      std::vector<Mode> v;
+      for (int k = 0; k < n; ++k, ++mpos) v.push_back({d->step_modes[3 * mpos], d->step_modes[3 * mpos + 1],
+                                                       d->step_modes[3 * mpos + 2]});
+      return v;
+    };
+    st.spec.M = merge_modes(take(nm));
+    st.spec.N = merge_modes(take(nn));
+    st.spec.K = merge_modes(take(nk));
+    for (auto* v : {&st.spec.M, &st.spec.N, &st.spec.K})
+      for (auto& m : *v)
+        if (m.ext < 1 || m.s0 < 0 || m.s1 < 0) return bad("bad mode");
+    // In-bounds check of every operand for one batch entry; a leaf must also
+    // hold every slice offset on top of the access span.
+    const Slot &A = p->slots[st.a], &B = p->slots[st.b], &Cs = p->slots[st.c];
+    auto fits = [&](const Slot& sl, int64_t span) -> bool {
+      if (!sl.leaf) return span < sl.elems;
+      Tensor* t = get(c, sl.leaf);
+      int64_t full = 0, extra = 0;
+      for (int q = 0; q < t->rank; ++q) full += (t->dims[q] - 1) * t->strides[q];
+      for (int64_t v : sl.slice_stride) extra += v;
+      return span + extra <= full;
+    };
+    if (!fits(A, span_of(st.spec.M, true) + span_of(st.spec.K, true))) return bad("A access out of range");
+    if (!fits(B, span_of(st.spec.N, true) + span_of(st.spec.K, false))) return bad("B access out of range");
+    if (!fits(Cs, span_of(st.spec.M, false) + span_of(st.spec.N, false))) return bad("C access out of range");
+    st.dep = Cs.batched;
+    if (!st.dep && (A.batched || B.batched)) return bad("unbatched output from a batched operand");
+    // Batch group over the slices of a group.
+    st.spec.L = single_batch();
+    if (st.dep && G > 1) {
+      Batch& L = st.spec.L;
+      L.nmodes = d->group_bits;
+      L.total = (uint32_t)G;
+      auto stride_of = [&](const Slot& sl, int j) -> int64_t {
+        if (sl.leaf) return sl.slice_stride[j];
+        return sl.batched ? sl.elems * ((int64_t)1 << j) : 0;
+      };
+      for (int j = 0; j < d->group_bits; ++j) {
+        L.ext[j] = make_fdiv(2);
+        L.sA[j] = stride_of(A, j);
+        L.sB[j] = stride_of(B, j);
+        L.sC[j] = stride_of(Cs, j);
+      }
+    }
+    st.spec.kc = d->step_kc ? d->step_kc[i] : -1;
+    st.spec.tile = d->step_tile ? d->step_tile[i] : -1;
+    int r = resolve_step(c, st.spec, c->esize);
+    if (r) {
+      std::string msg = c->err;
+      delete p;
+      return fail(c, r, "step " + std::to_string(i) + ": " + msg);
+    }
+    const double nbat = st.dep ? (double)G * p->n_groups : 1.0;
+    st.flops = 8.0 * st.spec.Mt * st.spec.Nt * st.spec.Kt * nbat;
+    st.bytes = (double)c->esize * (st.spec.Mt * st.spec.Kt + st.spec.Kt * st.spec.Nt + st.spec.Mt * st.spec.Nt) * nbat;
+    p->alg_flops += st.flops;
+    p->alg_bytes += st.bytes;
+    (st.dep ? p->order2 : p->order1).push_back(i);
+    p->scratch_bytes = std::max(p->scratch_bytes, step_scratch_bytes(st.spec, c->esize));
+  }
+  if (!p->slots[p->out_slot].leaf && produced[p->out_slot] < 0) return bad("output slot never produced");
+  if (d->n_sliced > 0 && !p->slots[p->out_slot].batched) return bad("sliced plan with a slice-independent output");
+  // Leaf bounds over all slice offsets.
+  for (auto& sl : p->slots) {
+    if (!sl.leaf) continue;
+    Tensor* t = get(c, sl.leaf);
+    int64_t extra = 0;
+    for (int64_t v : sl.slice_stride) {
+      if (v < 0) return bad("negative slice stride");
+      extra += v;
+    }
+    int64_t span = 0;
+    for (int i = 0; i < t->rank; ++i) span += (t->dims[i] - 1) * t->strides[i];
+    if (extra > span) return bad("slice strides exceed the leaf");
+  }
+  // Liveness over the sequence [phase 1][phase 2 of one group]; anything a
+  // phase-2 step reads stays live to the end (every group reads it again).
+  std::vector<int> pos(d->n_steps);
+  int t = 0;
+  for (int i : p->order1) pos[i] = t++;
+  const int phase2_start = t;
+  for (int i : p->order2) pos[i] = t++;
+  const int end = t;
+  struct Live { int s, start, stop; size_t bytes; };
+  std::vector<Live> lives;
+  for (int s = 0; s < d->n_slots; ++s) {
+    Slot& sl = p->slots[s];
+    if (sl.leaf) continue;
+    if (sl.producer < 0) continue;
+    int start = pos[sl.producer];
+    int stop = sl.last_use >= 0 ? pos[sl.last_use] : start;
+    if (s == p->out_slot) stop = end;
+    bool read_in_2 = false;
+    for (int i : p->order2)
+      if (p->steps[i].a == s || p->steps[i].b == s) read_in_2 = true;
+    if (read_in_2 && start < phase2_start) stop = end;
+    size_t bytes = (size_t)sl.elems * c->esize * (sl.batched ? G : 1);
+    bytes = (bytes + 255) / 256 * 256;
+    lives.push_back({s, start, stop, bytes});
+  }
+  // First-fit over time-ordered intervals.
+  std::sort(lives.begin(), lives.end(), [](const Live& x, const Live& y) { return x.start < y.start; });
+  std::vector<std::pair<size_t, size_t>> busy;  // (off, bytes) of live blocks
+  std::vector<Live> active;
+  size_t top = 0;
+  for (auto& L : lives) {
+    active.erase(std::remove_if(active.begin(), active.end(), [&](const Live& a) { return a.stop < L.start; }),
+                 active.end());
+    std::vector<std::pair<size_t, size_t>> used;
+    for (auto& a : active) used.push_back({(size_t)p->slots[a.s].arena_off, a.bytes});
+    std::sort(used.begin(), used.end());
+    size_t off = 0;
+    for (auto& u : used) {
+      if (off + L.bytes <= u.first) break;
+      off = std::max(off, u.first + u.second);
+    }
+    p->slots[L.s].arena_off = (int64_t)off;
+    top = std::max(top, off + L.bytes);
+    active.push_back(L);
+  }
+  p->arena_bytes = std::max<size_t>(top, 256);
+  if (hipMalloc(&p->arena, p->arena_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    delete p;
+    return fail(c, PQ_ERR_OOM, "plan arena allocation failed");
+  }
+  if (p->scratch_bytes && hipMalloc(&p->scratch, p->scratch_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    hipFree(p->arena);
+    delete p;
+    return fail(c, PQ_ERR_OOM, "plan scratch allocation failed");
+  }
+  p->out_elems = p->slots[p->out_slot].elems;
+  if (hipMalloc((void**)&p->out_dev, sizeof(double) * 2 * p->out_elems) != hipSuccess) {
+    (void)hipGetLastError();
+    hipFree(p->arena);
+    if (p->scratch) hipFree(p->scratch);
+    delete p;
+    return fail(c, PQ_ERR_OOM, "plan output allocation failed");
+  }
+  p->launches = 1 + (int64_t)p->order1.size() + p->n_groups * ((int64_t)p->order2.size() + 1);
+  *out = p;
+  return PQ_OK;
+}
+
+int pq_plan_execute(pq_ctx* c, pq_plan* p, int use_graph) {
+  if (!c || !p) return PQ_ERR_ARG;
+  if (!use_graph) return run_program(c, p, false);
+  if (!p->exec) {
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hipfail(c, e, "begin capture");
+    c->capturing = true;
+    int r = run_program(c, p, false);
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(c->stream, &g);
+    if (r) {
+      if (g) hipGraphDestroy(g);
+      return r;
+    }
+    if (e != hipSuccess) return hipfail(c, e, "end capture");
+    p->graph = g;
+    e = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) return hipfail(c, e, "graph instantiate");
+  }
+  hipError_t e = hipGraphLaunch(p->exec, c->stream);
+  if (e != hipSuccess) return hipfail(c, e, "graph launch");
+  return PQ_OK;
+}
+
+int pq_plan_result(pq_ctx* c, pq_plan* p, double* host) {
+  if (!c || !p || !host) return PQ_ERR_ARG;
+  hipError_t e = hipMemcpyAsync(host, p->out_dev, sizeof(double) * 2 * p->out_elems, hipMemcpyDeviceToHost,
+                                c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hipfail(c, e, "plan result");
+  return PQ_OK;
+}
+
+int pq_plan_stats(const pq_plan* p, int64_t* arena_bytes, int64_t* launches, double* alg_flops, double* alg_bytes,
+                  int64_t* out_elems) {
+  if (!p) return PQ_ERR_ARG;
+  if (arena_bytes) *arena_bytes = (int64_t)p->arena_bytes;
+  if (launches) *launches = p->launches;
+  if (alg_flops) *alg_flops = p->alg_flops;
+  if (alg_bytes) *alg_bytes = p->alg_bytes;
+  if (out_elems) *out_elems = p->out_elems;
+  return PQ_OK;
+}
+
+int pq_plan_step_info(const pq_plan* p, int step, int* kclass, int* tile, int* dep, double* flops, double* bytes) {
+  if (!p || step < 0 || step >= (int)p->steps.size()) return PQ_ERR_ARG;
+  const PStep& s = p->steps[step];
+  if (kclass) *kclass = s.spec.kc;
+  if (tile) *tile = s.spec.tile;
+  if (dep) *dep = s.dep ? 1 : 0;
+  if (flops) *flops = s.flops;
+  if (bytes) *bytes = s.bytes;
+  return PQ_OK;
+}
+
+int pq_plan_profile(pq_ctx* c, pq_plan* p, double* step_ms) {
+  if (!c || !p || !step_ms) return PQ_ERR_ARG;
+  const size_t n = p->steps.size();
+  if (p->ev.empty()) {
+    p->ev.resize(2 * n);
+    for (auto& e : p->ev) hipEventCreate(&e);
+  }
+  for (size_t i = 0; i < n; ++i) step_ms[i] = 0;
+  // Phase 2 steps run once per group: time them group by group.
+  hipStreamSynchronize(c->stream);
+  int r = PQ_OK;
+  {
+    // phase 1 and the first group, with events around every step
+    std::vector<int> o2 = p->order2;
+    int64_t ng = p->n_groups;
+    p->n_groups = 1;
+    r = run_program(c, p, true);
+    p->n_groups = ng;
+    if (r) return r;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hipfail(c, e, "profile sync");
+    for (size_t i = 0; i < n; ++i) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, p->ev[2 * i], p->ev[2 * i + 1]) == hipSuccess)
+        step_ms[i] = p->steps[i].dep ? ms * (double)ng : ms;
+    }
+  }
+  return r;
+}
+
+int pq_plan_destroy(pq_ctx* c, pq_plan* p) {
+  if (!p) return PQ_ERR_ARG;
+  if (c) hipStreamSynchronize(c->stream);
+  if (p->exec) hipGraphExecDestroy(p->exec);
+  if (p->graph) hipGraphDestroy(p->graph);
+  for (auto& e : p->ev) hipEventDestroy(e);
+  if (p->arena) hipFree(p->arena);
+  if (p->scratch) hipFree(p->scratch);
+  if (p->out_dev) hipFree(p->out_dev);
+  delete p;
+  return PQ_OK;
+}
+
+}  // extern "C"

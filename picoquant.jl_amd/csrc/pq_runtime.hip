@@ -2,37 +2,12 @@
 // caching allocator, the tensor table (refcounted buffers + strided views),
 // the lowering of an ncon-labelled pairwise contraction to index groups, and
 // the kernel dispatch.  The C ABI is declared in include/pq_hip.h.
-#include "pq_internal.h"
-#include "../../include/pq_hip.h"
+#include "pq_rt.h"
 
 #include <algorithm>
 #include <cstring>
-#include <map>
-#include <string>
-#include <unordered_map>
-#include <vector>
 
-namespace {
-
-struct Buf {
-  void* ptr = nullptr;
-  size_t bytes = 0;
-  int refs = 0;
-};
-
-struct Tensor {
-  Buf* buf = nullptr;
-  int64_t off = 0;  // element offset into buf
-  int rank = 0;
-  int64_t dims[PQ_MAX_RANK];
-  int64_t strides[PQ_MAX_RANK];
-};
-
-struct ProfEntry {
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
-  int64_t launches = 0;
-  double bytes = 0, flops = 0;
-};
+namespace pqrt {
 
 int64_t numel(const Tensor& t) {
   int64_t n = 1;
@@ -57,12 +32,6 @@ void set_contiguous(Tensor& t) {
   }
 }
 
-// A mode before lowering: extent and strides in up to two tensors.
-struct Mode {
-  int64_t ext;
-  int64_t s0, s1;
-};
-
 // Drop unit extents and merge neighbours that are contiguous in both tensors.
 std::vector<Mode> merge_modes(const std::vector<Mode>& in) {
   std::vector<Mode> out;
@@ -86,6 +55,7 @@ bool fill_group(Group& g, const std::vector<Mode>& modes) {
   g.nmodes = (int)modes.size();
   uint64_t total = 1;
   for (size_t j = 0; j < modes.size(); ++j) {
+    if (modes[j].s0 >= (1ll << 31) || modes[j].s1 >= (1ll << 31)) return false;
     g.ext[j] = make_fdiv((uint32_t)modes[j].ext);
     g.s0[j] = (int32_t)modes[j].s0;
     g.s1[j] = (int32_t)modes[j].s1;
@@ -96,27 +66,12 @@ bool fill_group(Group& g, const std::vector<Mode>& modes) {
   return true;
 }
 
-}  // namespace
-
-struct pq_ctx {
-  int device = 0;
-  int dtype = PQ_C128;
-  size_t esize = 16;
-  hipStream_t stream = nullptr;
-  std::string err;
-  std::unordered_map<int64_t, Tensor> tensors;
-  int64_t next_id = 1;
-  std::multimap<size_t, void*> free_blocks;
-  size_t cached = 0;
-  bool prof = false;
-  ProfEntry pe[PQ_KC_COUNT];
-  int override_kc = -1;
-  int gemm_tile = -1;
-  void* scratch = nullptr;  // split-K partials
-  size_t scratch_bytes = 0;
-};
-
-namespace {
+Batch single_batch() {
+  Batch L;
+  memset(&L, 0, sizeof(L));
+  L.total = 1;
+  return L;
+}
 
 int fail(pq_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -127,7 +82,7 @@ int hipfail(pq_ctx* c, hipError_t e, const char* where) {
   return fail(c, PQ_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
 }
 
-size_t round_bytes(size_t b) {
+static size_t round_bytes(size_t b) {
   if (b < 256) return 256;
   if (b < (2u << 20)) {
     size_t p = 256;
@@ -200,28 +155,25 @@ pq_tensor add(pq_ctx* c, const Tensor& t) {
   return id;
 }
 
-struct ProfScope {
-  pq_ctx* c;
-  int kc;
-  hipEvent_t e1 = nullptr;
-  ProfScope(pq_ctx* c_, int kc_, double bytes, double flops) : c(c_), kc(kc_) {
-    if (!c->prof) return;
-    hipEvent_t e0;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventRecord(e0, c->stream);
-    c->pe[kc].events.push_back({e0, e1});
-    c->pe[kc].launches++;
-    c->pe[kc].bytes += bytes;
-    c->pe[kc].flops += flops;
-  }
-  ~ProfScope() {
-    if (c->prof) hipEventRecord(e1, c->stream);
-  }
-};
+ProfScope::ProfScope(pq_ctx* c_, int kc_, double bytes, double flops) : c(c_), kc(kc_) {
+  on = c->prof && !c->capturing;
+  if (!on) return;
+  hipEvent_t e0;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, c->stream);
+  c->pe[kc].events.push_back({e0, e1});
+  c->pe[kc].launches++;
+  c->pe[kc].bytes += bytes;
+  c->pe[kc].flops += flops;
+}
+
+ProfScope::~ProfScope() {
+  if (on) hipEventRecord(e1, c->stream);
+}
 
 template <typename C>
-C* dptr(const Tensor& t) {
+static C* dptr(const Tensor& t) {
   return reinterpret_cast<C*>(t.buf->ptr) + t.off;
 }
 
@@ -238,7 +190,10 @@ int materialize_into(pq_ctx* c, const Tensor& src, Tensor& dst) {
   for (int i = 0; i < src.rank; ++i) modes.push_back({src.dims[i], src.strides[i], dst.strides[i]});
   modes = merge_modes(modes);
   CopyArgs a;
-  if (!fill_group(a.G, modes)) return fail(c, PQ_ERR_UNSUPPORTED, "tensor too large or too fragmented");
+  if (!fill_group(a.G, modes)) {
+    unref(c, dst.buf);
+    return fail(c, PQ_ERR_UNSUPPORTED, "tensor too large or too fragmented");
+  }
   ProfScope ps(c, PQ_KC_COPY, 2.0 * n * c->esize, 0);
   hipError_t e = (c->dtype == PQ_C128)
                      ? launch_copy<c128>(dptr<c128>(src), dptr<c128>(dst), a, c->stream)
@@ -249,7 +204,7 @@ int materialize_into(pq_ctx* c, const Tensor& src, Tensor& dst) {
 
 // Default workgroup tile for the MFMA GEMM: big tiles when both free extents
 // fill them, narrow ones otherwise (complex128 is limited to 64x64).
-int pick_tile(uint64_t Mt, uint64_t Nt, size_t esize) {
+static int pick_tile(uint64_t Mt, uint64_t Nt, size_t esize) {
   if (Mt <= 32 && Nt <= 32) return 0;
   if (Nt <= 32) return 1;
   if (Mt <= 32) return 2;
@@ -258,115 +213,118 @@ int pick_tile(uint64_t Mt, uint64_t Nt, size_t esize) {
   return 3;
 }
 
-template <typename C>
-int run_contract(pq_ctx* c, const Tensor& A, const Tensor& B, Tensor& Ct,
-                 const std::vector<Mode>& Mm, const std::vector<Mode>& Km,
-                 const std::vector<Mode>& Nm) {
-  // Mm: (ext, sA, sC), Km: (ext, sA, sB), Nm: (ext, sB, sC) -- already merged.
-  uint64_t Mt = 1, Nt = 1, Kt = 1;
-  for (auto& m : Mm) Mt *= m.ext;
-  for (auto& m : Nm) Nt *= m.ext;
-  for (auto& m : Km) Kt *= m.ext;
-  const C* pa = dptr<C>(A);
-  const C* pb = dptr<C>(B);
-  C* pc = dptr<C>(Ct);
-  const double es = (double)sizeof(C);
-  const double alg_bytes = es * (double)(Mt * Kt + Kt * Nt + Mt * Nt);
-  const double alg_flops = 8.0 * (double)Mt * (double)Nt * (double)Kt;
-  int kc = c->override_kc;
+static bool stream_fits(uint64_t Kt, uint64_t Ny, size_t esize) {
+  uint64_t KP = 1;
+  while (KP < Kt) KP <<= 1;
+  size_t smem = KP * Ny * esize + KP * 4 + Ny * 4;
+  return Kt <= PQ_STREAM_KMAX && Kt * Ny <= PQ_STREAM_YMAX && smem <= 48 * 1024;
+}
 
-  auto stream_ok_AB = [&]() {  // X = A big, Y = B small
-    size_t smem = 0;
-    uint64_t KP = 1;
-    while (KP < Kt) KP <<= 1;
-    smem = KP * Nt * sizeof(C) + KP * 4 + Nt * 4;
-    return Kt <= PQ_STREAM_KMAX && Kt * Nt <= PQ_STREAM_YMAX && smem <= 48 * 1024;
-  };
-  auto stream_ok_BA = [&]() {
-    uint64_t KP = 1;
-    while (KP < Kt) KP <<= 1;
-    size_t smem = KP * Mt * sizeof(C) + KP * 4 + Mt * 4;
-    return Kt <= PQ_STREAM_KMAX && Kt * Mt <= PQ_STREAM_YMAX && smem <= 48 * 1024;
-  };
-
+int resolve_step(pq_ctx* c, StepSpec& s, size_t esize) {
+  s.Mt = s.Nt = s.Kt = 1;
+  for (auto& m : s.M) s.Mt *= m.ext;
+  for (auto& m : s.N) s.Nt *= m.ext;
+  for (auto& m : s.K) s.Kt *= m.ext;
+  const uint64_t Mt = s.Mt, Nt = s.Nt, Kt = s.Kt;
+  const bool ab = stream_fits(Kt, Nt, esize), ba = stream_fits(Kt, Mt, esize);
   const bool gemm_ok = Kt <= (uint64_t)gemm_k_limit() && Mt >= 16 && Nt >= 16 && Kt >= 4;
+  int kc = s.kc >= 0 ? s.kc : c->override_kc;
   if (kc < 0) {
-    if ((stream_ok_AB() && Kt * Nt <= 64) || (stream_ok_BA() && Kt * Mt <= 64)) kc = PQ_KC_STREAM;
+    if ((ab && Kt * Nt <= 64) || (ba && Kt * Mt <= 64)) kc = PQ_KC_STREAM;
     else if (Mt * Nt <= 16 && Kt > 64) kc = PQ_KC_REDUCE;
     else if (gemm_ok) kc = PQ_KC_GEMM;
-    else if (stream_ok_AB() || stream_ok_BA()) kc = PQ_KC_STREAM;
+    else if (ab || ba) kc = PQ_KC_STREAM;
     else kc = PQ_KC_GENERIC;
   }
-
-  hipError_t e = hipSuccess;
   if (kc == PQ_KC_STREAM) {
+    if (!ab && !ba) return fail(c, PQ_ERR_UNSUPPORTED, "stream kernel not applicable");
+    s.stream_ab = ab && (Mt >= Nt || !ba);
+  } else if (kc == PQ_KC_GEMM) {
+    if (!gemm_ok) return fail(c, PQ_ERR_UNSUPPORTED, "gemm kernel not applicable");
+    int t = s.tile >= 0 ? s.tile : c->gemm_tile;
+    s.tile = t >= 0 ? t : pick_tile(Mt, Nt, esize);
+  } else if (kc == PQ_KC_REDUCE) {
+    if (Mt * Nt > 16) return fail(c, PQ_ERR_UNSUPPORTED, "reduce kernel needs M*N <= 16");
+  } else if (kc != PQ_KC_GENERIC) {
+    return fail(c, PQ_ERR_UNSUPPORTED, "kernel class not available for this shape");
+  }
+  if (s.L.total < 1 || s.L.total > 65535) return fail(c, PQ_ERR_UNSUPPORTED, "batch too large");
+  s.kc = kc;
+  return PQ_OK;
+}
+
+size_t step_scratch_bytes(const StepSpec& s, size_t esize) {
+  if (s.kc != PQ_KC_REDUCE) return 0;
+  return (size_t)reduce_blocks_for((uint32_t)s.Kt) * s.Mt * s.Nt * s.L.total * esize;
+}
+
+template <typename C>
+static int launch_typed(pq_ctx* c, const C* pa, const C* pb, C* pc, const StepSpec& s, void* scratch) {
+  const double es = (double)sizeof(C);
+  const double nb = (double)s.L.total;
+  const double alg_bytes = nb * es * (double)(s.Mt * s.Kt + s.Kt * s.Nt + s.Mt * s.Nt);
+  const double alg_flops = nb * 8.0 * (double)s.Mt * (double)s.Nt * (double)s.Kt;
+  hipError_t e = hipSuccess;
+  if (s.kc == PQ_KC_STREAM) {
     StreamArgs sa;
-    bool ab = stream_ok_AB() && (Mt >= Nt || !stream_ok_BA());
-    if (!ab && !stream_ok_BA()) return fail(c, PQ_ERR_UNSUPPORTED, "stream kernel not applicable");
-    if (ab) {
-      if (!fill_group(sa.M, Mm) || !fill_group(sa.K, Km) || !fill_group(sa.N, Nm))
+    memset(&sa, 0, sizeof(sa));
+    sa.L = s.L;
+    if (s.stream_ab) {
+      if (!fill_group(sa.M, s.M) || !fill_group(sa.K, s.K) || !fill_group(sa.N, s.N))
         return fail(c, PQ_ERR_UNSUPPORTED, "too many modes");
       ProfScope ps(c, PQ_KC_STREAM, alg_bytes, alg_flops);
       e = launch_stream<C>(pa, pb, pc, sa, c->stream);
     } else {
-      // X = B (big), Y = A (small): K strides swap roles.
+      // X = B (big), Y = A (small): K and batch strides swap roles.
       std::vector<Mode> K2;
-      for (auto& m : Km) K2.push_back({m.ext, m.s1, m.s0});
-      if (!fill_group(sa.M, Nm) || !fill_group(sa.K, K2) || !fill_group(sa.N, Mm))
+      for (auto& m : s.K) K2.push_back({m.ext, m.s1, m.s0});
+      for (int j = 0; j < sa.L.nmodes; ++j) std::swap(sa.L.sA[j], sa.L.sB[j]);
+      if (!fill_group(sa.M, s.N) || !fill_group(sa.K, K2) || !fill_group(sa.N, s.M))
         return fail(c, PQ_ERR_UNSUPPORTED, "too many modes");
       ProfScope ps(c, PQ_KC_STREAM, alg_bytes, alg_flops);
       e = launch_stream<C>(pb, pa, pc, sa, c->stream);
     }
-  } else if (kc == PQ_KC_REDUCE) {
-    if (Mt * Nt > 16) return fail(c, PQ_ERR_UNSUPPORTED, "reduce kernel needs M*N <= 16");
-    GenArgs ga;
-    if (!fill_group(ga.M, Mm) || !fill_group(ga.N, Nm) || !fill_group(ga.K, Km))
-      return fail(c, PQ_ERR_UNSUPPORTED, "too many modes");
-    int nb = reduce_blocks_for((uint32_t)Kt);
-    size_t need = (size_t)nb * Mt * Nt * sizeof(C);
-    if (c->scratch_bytes < need) {
-      if (c->scratch) {
-        hipStreamSynchronize(c->stream);
-        hipFree(c->scratch);
-        c->scratch = nullptr;
-        c->scratch_bytes = 0;
-      }
-      if (hipMalloc(&c->scratch, need) != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(c, PQ_ERR_OOM, "scratch allocation failed");
-      }
-      c->scratch_bytes = need;
-    }
-    ProfScope ps(c, PQ_KC_REDUCE, alg_bytes, alg_flops);
-    e = launch_reduce<C>(pa, pb, pc, ga, reinterpret_cast<C*>(c->scratch), nb, c->stream);
-  } else if (kc == PQ_KC_GEMM) {
-    if (!gemm_ok) return fail(c, PQ_ERR_UNSUPPORTED, "gemm kernel not applicable");
+  } else if (s.kc == PQ_KC_GEMM) {
     GemmArgs ga;
     memset(&ga, 0, sizeof(ga));
-    if (!fill_group(ga.M, Mm) || !fill_group(ga.N, Nm) || !fill_group(ga.K, Km))
+    if (!fill_group(ga.M, s.M) || !fill_group(ga.N, s.N) || !fill_group(ga.K, s.K))
       return fail(c, PQ_ERR_UNSUPPORTED, "too many modes");
-    ga.batch = 1;
-    for (auto& m : Km) {
+    ga.L = s.L;
+    for (auto& m : s.K) {
       if (m.s0 == 1) ga.a_kfast = 1;
       if (m.s1 == 1) ga.b_kfast = 1;
     }
-    ga.tile = c->gemm_tile >= 0 ? c->gemm_tile : pick_tile(Mt, Nt, sizeof(C));
+    ga.tile = s.tile;
     ProfScope ps(c, PQ_KC_GEMM, alg_bytes, alg_flops);
     e = launch_gemm<C>(pa, pb, pc, ga, c->stream);
-  } else if (kc == PQ_KC_GENERIC) {
-    GenArgs ga;
-    if (!fill_group(ga.M, Mm) || !fill_group(ga.N, Nm) || !fill_group(ga.K, Km))
-      return fail(c, PQ_ERR_UNSUPPORTED, "too many modes");
-    ProfScope ps(c, PQ_KC_GENERIC, alg_bytes, alg_flops);
-    e = launch_generic<C>(pa, pb, pc, ga, c->stream);
   } else {
-    return fail(c, PQ_ERR_UNSUPPORTED, "kernel class not available for this shape");
+    GenArgs ga;
+    memset(&ga, 0, sizeof(ga));
+    if (!fill_group(ga.M, s.M) || !fill_group(ga.N, s.N) || !fill_group(ga.K, s.K))
+      return fail(c, PQ_ERR_UNSUPPORTED, "too many modes");
+    ga.L = s.L;
+    if (s.kc == PQ_KC_REDUCE) {
+      ProfScope ps(c, PQ_KC_REDUCE, alg_bytes, alg_flops);
+      e = launch_reduce<C>(pa, pb, pc, ga, reinterpret_cast<C*>(scratch), reduce_blocks_for((uint32_t)s.Kt),
+                           c->stream);
+    } else {
+      ProfScope ps(c, PQ_KC_GENERIC, alg_bytes, alg_flops);
+      e = launch_generic<C>(pa, pb, pc, ga, c->stream);
+    }
   }
   if (e != hipSuccess) return hipfail(c, e, "contraction kernel");
   return PQ_OK;
 }
 
-}  // namespace
+int launch_step(pq_ctx* c, const void* A, const void* B, void* Out, const StepSpec& s, void* scratch) {
+  if (c->dtype == PQ_C128)
+    return launch_typed<c128>(c, (const c128*)A, (const c128*)B, (c128*)Out, s, scratch);
+  return launch_typed<c64>(c, (const c64*)A, (const c64*)B, (c64*)Out, s, scratch);
+}
+
+}  // namespace pqrt
+
+using namespace pqrt;
 
 extern "C" {
 
@@ -515,7 +473,6 @@ int pq_contract(pq_ctx* c, pq_tensor ha, int ra, const int32_t* ia, pq_tensor hb
   if (!A || !B) return fail(c, PQ_ERR_NOT_FOUND, "unknown tensor handle");
   if (ra != A->rank || rb != B->rank) return fail(c, PQ_ERR_ARG, "label count does not match tensor rank");
   if ((ra && !ia) || (rb && !ib)) return fail(c, PQ_ERR_ARG, "null label array");
-  // Validate labels: positive labels appear once in A and once in B.
   for (int i = 0; i < ra; ++i) {
     if (ia[i] == 0) return fail(c, PQ_ERR_ARG, "label 0 is not allowed");
     for (int j = i + 1; j < ra; ++j)
@@ -526,8 +483,6 @@ int pq_contract(pq_ctx* c, pq_tensor ha, int ra, const int32_t* ia, pq_tensor hb
     for (int j = i + 1; j < rb; ++j)
       if (ib[i] == ib[j]) return fail(c, PQ_ERR_ARG, "repeated label in B");
   }
-  Tensor Ct;
-  Ct.rank = 0;
   std::vector<int> a_open, b_open;
   std::vector<std::pair<int, int>> common;
   for (int i = 0; i < ra; ++i) {
@@ -545,8 +500,9 @@ int pq_contract(pq_ctx* c, pq_tensor ha, int ra, const int32_t* ia, pq_tensor hb
     }
   }
   for (int j = 0; j < rb; ++j) {
-    if (ib[j] < 0) b_open.push_back(j);
-    else {
+    if (ib[j] < 0) {
+      b_open.push_back(j);
+    } else {
       bool found = false;
       for (int i = 0; i < ra; ++i)
         if (ia[i] == ib[j]) found = true;
@@ -555,6 +511,7 @@ int pq_contract(pq_ctx* c, pq_tensor ha, int ra, const int32_t* ia, pq_tensor hb
   }
   int rc = (int)(a_open.size() + b_open.size());
   if (rc > PQ_MAX_RANK) return fail(c, PQ_ERR_UNSUPPORTED, "result rank too large");
+  Tensor Ct;
   Ct.rank = rc;
   int p = 0;
   for (int i : a_open) Ct.dims[p++] = A->dims[i];
@@ -573,22 +530,39 @@ int pq_contract(pq_ctx* c, pq_tensor ha, int ra, const int32_t* ia, pq_tensor hb
       return hipfail(c, e, "memset");
     }
   } else if (nc > 0) {
-    std::vector<Mode> Mm, Km, Nm;
+    StepSpec s;
     p = 0;
-    for (int i : a_open) {
-      Mm.push_back({A->dims[i], A->strides[i], Ct.strides[p]});
-      ++p;
+    for (int i : a_open) s.M.push_back({A->dims[i], A->strides[i], Ct.strides[p++]});
+    for (int j : b_open) s.N.push_back({B->dims[j], B->strides[j], Ct.strides[p++]});
+    for (auto& pr : common) s.K.push_back({A->dims[pr.first], A->strides[pr.first], B->strides[pr.second]});
+    s.M = merge_modes(s.M);
+    s.N = merge_modes(s.N);
+    s.K = merge_modes(s.K);
+    s.L = single_batch();
+    int r = resolve_step(c, s, c->esize);
+    if (r == PQ_OK) {
+      size_t need = step_scratch_bytes(s, c->esize);
+      if (need > c->scratch_bytes) {
+        if (c->scratch) {
+          hipStreamSynchronize(c->stream);
+          hipFree(c->scratch);
+          c->scratch = nullptr;
+          c->scratch_bytes = 0;
+        }
+        if (hipMalloc(&c->scratch, need) != hipSuccess) {
+          (void)hipGetLastError();
+          r = fail(c, PQ_ERR_OOM, "scratch allocation failed");
+        } else {
+          c->scratch_bytes = need;
+        }
+      }
     }
-    for (int j : b_open) {
-      Nm.push_back({B->dims[j], B->strides[j], Ct.strides[p]});
-      ++p;
+    if (r == PQ_OK) {
+      const unsigned char* pa = (const unsigned char*)A->buf->ptr + A->off * c->esize;
+      const unsigned char* pb = (const unsigned char*)B->buf->ptr + B->off * c->esize;
+      unsigned char* pc = (unsigned char*)Ct.buf->ptr;
+      r = launch_step(c, pa, pb, pc, s, c->scratch);
     }
-    for (auto& pr : common) Km.push_back({A->dims[pr.first], A->strides[pr.first], B->strides[pr.second]});
-    Mm = merge_modes(Mm);
-    Nm = merge_modes(Nm);
-    Km = merge_modes(Km);
-    int r = (c->dtype == PQ_C128) ? run_contract<c128>(c, *A, *B, Ct, Mm, Km, Nm)
-                                  : run_contract<c64>(c, *A, *B, Ct, Mm, Km, Nm);
     if (r) {
       unref(c, Ct.buf);
       return r;

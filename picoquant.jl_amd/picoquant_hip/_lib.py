@@ -53,6 +53,14 @@ SIGNATURES = {
                                    ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
     "pq_set_kernel_override": (ct.c_int, [_P, ct.c_int]),
     "pq_set_gemm_tile": (ct.c_int, [_P, ct.c_int]),
+    "pq_plan_create": (ct.c_int, [_P, _P, ct.POINTER(_P)]),
+    "pq_plan_execute": (ct.c_int, [_P, _P, ct.c_int]),
+    "pq_plan_result": (ct.c_int, [_P, _P, ct.POINTER(ct.c_double)]),
+    "pq_plan_stats": (ct.c_int, [_P, _I64P, _I64P, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double), _I64P]),
+    "pq_plan_step_info": (ct.c_int, [_P, ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int),
+                                     ct.POINTER(ct.c_int), ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
+    "pq_plan_profile": (ct.c_int, [_P, _P, ct.POINTER(ct.c_double)]),
+    "pq_plan_destroy": (ct.c_int, [_P, _P]),
 }
 
 _lib = None

@@ -1,0 +1,264 @@
+"""Planned execution of a whole contraction (all slices, FP64 slice sum) on
+libpq_hip's plan executor (pq_plan_* in include/pq_hip.h).
+
+compile_plan() lowers (network, pair plan, sliced bonds) to slots and steps:
+  * leaves are the network's device tensors; a sliced bond stays in the leaf
+    and is selected per slice through its stride;
+  * an intermediate is laid out with the indices its consumer sums first
+    (so GEMM operand loads run along K), the rest after; the final tensor is
+    laid out in output-qubit order, which is the reference's final permute
+    (src/layer2.jl:306-316);
+  * a slot that depends on a sliced bond is batched over the slices of a
+    slice group; everything else runs once.
+The result is the sum over slices, accumulated in FP64 on the device.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import math
+
+import numpy as np
+
+from . import _lib
+
+
+class PlanDesc(ct.Structure):
+    _fields_ = [
+        ("n_slots", ct.c_int32),
+        ("slot_elems", ct.POINTER(ct.c_int64)),
+        ("slot_batched", ct.POINTER(ct.c_int32)),
+        ("slot_leaf", ct.POINTER(ct.c_int64)),
+        ("n_sliced", ct.c_int32),
+        ("leaf_slice_stride", ct.POINTER(ct.c_int64)),
+        ("group_bits", ct.c_int32),
+        ("n_steps", ct.c_int32),
+        ("step_slots", ct.POINTER(ct.c_int32)),
+        ("step_kc", ct.POINTER(ct.c_int32)),
+        ("step_tile", ct.POINTER(ct.c_int32)),
+        ("step_nmodes", ct.POINTER(ct.c_int32)),
+        ("step_modes", ct.POINTER(ct.c_int64)),
+        ("out_slot", ct.c_int32),
+    ]
+
+
+def _arr(ctype, xs):
+    return (ctype * max(1, len(xs)))(*xs)
+
+
+class SlotInfo:
+    __slots__ = ("idx", "ext", "stride", "leaf", "dep", "slice_stride", "elems")
+
+    def __init__(self):
+        self.idx = []          # per-slice indices, in layout order
+        self.ext = {}          # index -> extent
+        self.stride = {}       # index -> per-slice stride
+        self.leaf = 0
+        self.dep = False
+        self.slice_stride = []
+        self.elems = 1
+
+
+def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf_strides=None):
+    """Pure-host lowering (no device calls): returns (slots, steps, out_slot)
+    where steps = [(a, b, c, M, N, K)] with modes as (ext, s0, s1)."""
+    sliced = list(sliced)
+    sset = set(sliced)
+    slots: list[SlotInfo] = []
+    label_slot = {}
+    for lab, node in network.nodes.items():
+        s = SlotInfo()
+        dims = list(node.dims)
+        if leaf_strides is not None:
+            strides = list(leaf_strides[lab])
+        else:
+            strides = [int(np.prod(dims[:i])) for i in range(len(dims))]
+        s.leaf = leaf_handles[lab] if leaf_handles is not None else 0
+        s.slice_stride = [0] * len(sliced)
+        for ix, d, st in zip(node.indices, dims, strides):
+            if ix in sset:
+                if d != 2:
+                    raise ValueError(f"sliced bond {ix} has extent {d}; only extent 2 is supported")
+                s.slice_stride[sliced.index(ix)] = st
+                s.dep = True
+            else:
+                s.idx.append(ix)
+                s.ext[ix] = d
+                s.stride[ix] = st
+        s.elems = int(np.prod(dims)) if dims else 1
+        label_slot[lab] = len(slots)
+        slots.append(s)
+    # Pass 1: index sets of every intermediate, in reference order.
+    counter = network.counters["node"]
+    raw = []
+    for a, b in plan:
+        sa, sb = label_slot.pop(a), label_slot.pop(b)
+        A, B = slots[sa], slots[sb]
+        bset, aset = set(B.idx), set(A.idx)
+        c_idx = [x for x in A.idx if x not in bset] + [x for x in B.idx if x not in aset]
+        C = SlotInfo()
+        C.idx = c_idx
+        C.ext = {x: (A.ext.get(x) or B.ext.get(x)) for x in c_idx}
+        C.dep = A.dep or B.dep
+        counter += 1
+        label_slot[f"node_{counter}"] = len(slots)
+        raw.append((sa, sb, len(slots)))
+        slots.append(C)
+    if len(label_slot) != 1:
+        raise ValueError(f"plan leaves {len(label_slot)} tensors uncontracted")
+    (out_slot,) = label_slot.values()
+    # Pass 2: layouts.  Consumer of each intermediate -> indices summed there.
+    consumer = {}
+    for sa, sb, sc in raw:
+        consumer[sa] = sb
+        consumer[sb] = sa
+    for sa, sb, sc in raw:
+        C = slots[sc]
+        if sc == out_slot:
+            order = out_order if out_order is not None else [
+                x for x in network.output_qubits if x in C.ext]
+            if sorted(order) != sorted(C.idx):
+                raise ValueError("output order does not match the open indices")
+            C.idx = list(order)
+        elif sc in consumer:
+            partner = set(slots[consumer[sc]].idx) if consumer[sc] < sc else None
+            if partner is None:
+                # partner produced later: its index set is known from pass 1
+                partner = set(slots[consumer[sc]].idx)
+            first = [x for x in C.idx if x in partner]
+            C.idx = first + [x for x in C.idx if x not in partner]
+        st = 1
+        for x in C.idx:
+            C.stride[x] = st
+            st *= C.ext[x]
+        C.elems = st
+    # Pass 3: modes.
+    steps = []
+    for sa, sb, sc in raw:
+        A, B, C = slots[sa], slots[sb], slots[sc]
+        bset, aset = set(B.idx), set(A.idx)
+        M = sorted([(A.ext[x], A.stride[x], C.stride[x]) for x in A.idx if x not in bset], key=lambda m: m[2])
+        N = sorted([(B.ext[x], B.stride[x], C.stride[x]) for x in B.idx if x not in aset], key=lambda m: m[2])
+        K = sorted([(A.ext[x], A.stride[x], B.stride[x]) for x in A.idx if x in bset], key=lambda m: m[1])
+        steps.append((sa, sb, sc, M, N, K))
+    return slots, steps, out_slot
+
+
+class Plan:
+    """A compiled plan on a context.  execute() enqueues one full run."""
+
+    def __init__(self, ctx, ptr, slots, steps, out_slot, n_sliced, group_bits):
+        self.ctx = ctx
+        self.ptr = ptr
+        self.slots = slots
+        self.steps = steps
+        self.out_slot = out_slot
+        self.n_sliced = n_sliced
+        self.group_bits = group_bits
+        out = slots[out_slot]
+        self.out_shape = tuple(out.ext[x] for x in out.idx)
+
+    def execute(self, use_graph=True):
+        self.ctx.check(self.ctx.lib.pq_plan_execute(self.ctx.ptr, self.ptr, 1 if use_graph else 0))
+
+    def result(self):
+        n = int(np.prod(self.out_shape)) if self.out_shape else 1
+        buf = np.empty(2 * n, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.pq_plan_result(self.ctx.ptr, self.ptr,
+                                                   buf.ctypes.data_as(ct.POINTER(ct.c_double))))
+        return (buf[0::2] + 1j * buf[1::2]).reshape(self.out_shape, order="F")
+
+    def stats(self):
+        a, l, o = ct.c_int64(), ct.c_int64(), ct.c_int64()
+        f, b = ct.c_double(), ct.c_double()
+        self.ctx.check(self.ctx.lib.pq_plan_stats(self.ptr, ct.byref(a), ct.byref(l), ct.byref(f), ct.byref(b),
+                                                  ct.byref(o)))
+        return {"arena_bytes": a.value, "launches": l.value, "alg_flops": f.value, "alg_bytes": b.value,
+                "out_elems": o.value}
+
+    def step_info(self, i):
+        kc, tile, dep = ct.c_int(), ct.c_int(), ct.c_int()
+        f, b = ct.c_double(), ct.c_double()
+        self.ctx.check(self.ctx.lib.pq_plan_step_info(self.ptr, i, ct.byref(kc), ct.byref(tile), ct.byref(dep),
+                                                      ct.byref(f), ct.byref(b)))
+        return {"kc": kc.value, "tile": tile.value, "dep": bool(dep.value), "flops": f.value, "bytes": b.value}
+
+    def profile(self):
+        ms = np.zeros(len(self.steps), dtype=np.float64)
+        self.ctx.check(self.ctx.lib.pq_plan_profile(self.ctx.ptr, self.ptr, ms.ctypes.data_as(ct.POINTER(ct.c_double))))
+        return ms
+
+    def close(self):
+        if self.ptr:
+            self.ctx.lib.pq_plan_destroy(self.ctx.ptr, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _leaf_layout(backend):
+    handles, strides = {}, {}
+    ctx = backend.ctx
+    for lab, h in backend.tensors.items():
+        base = ct.c_void_p()
+        off = ct.c_int64()
+        st = (ct.c_int64 * _lib.PQ_MAX_RANK)()
+        ctx.check(ctx.lib.pq_tensor_device_ptr(ctx.ptr, h, ct.byref(base), ct.byref(off), st))
+        n = len(ctx.shape(h))
+        handles[lab] = h
+        strides[lab] = [int(st[i]) for i in range(n)]
+    return handles, strides
+
+
+def choose_group_bits(slots, n_sliced, budget_bytes, esize):
+    """Largest slice group whose batched intermediates fit the budget."""
+    peak = max((s.elems for s in slots if s.dep and not s.leaf), default=1)
+    gb = n_sliced
+    while gb > 0 and 3 * peak * esize * (1 << gb) > budget_bytes:
+        gb -= 1
+    return min(gb, 16)
+
+
+def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=None, tiles=None,
+                 budget_bytes=64 << 30):
+    """Compile a pair plan of a network whose backend is a HIPBackend."""
+    backend = network.backend
+    ctx = backend.ctx
+    handles, strides = _leaf_layout(backend)
+    missing = [lab for lab in network.nodes if lab not in handles]
+    if missing:
+        raise KeyError(f"leaf tensors missing on the device: {missing[:4]}")
+    slots, steps, out_slot = lower_plan(network, plan, sliced, out_order, handles, strides)
+    n_sliced = len(list(sliced))
+    esize = 16 if backend.dtype == np.complex128 else 8
+    if group_bits is None:
+        group_bits = choose_group_bits(slots, n_sliced, budget_bytes, esize)
+    slot_elems = [s.elems for s in slots]
+    slot_batched = [1 if (s.dep and not s.leaf) else 0 for s in slots]
+    slot_leaf = [s.leaf for s in slots]
+    lss = []
+    for s in slots:
+        lss += (s.slice_stride if s.leaf else [0] * n_sliced)
+    step_slots, nmodes, modes = [], [], []
+    for sa, sb, sc, M, N, K in steps:
+        step_slots += [sa, sb, sc]
+        nmodes += [len(M), len(N), len(K)]
+        for grp in (M, N, K):
+            for m in grp:
+                modes += list(m)
+    kcs = list(kc) if kc is not None else [-1] * len(steps)
+    tls = list(tiles) if tiles is not None else [-1] * len(steps)
+    keep = dict(
+        slot_elems=_arr(ct.c_int64, slot_elems), slot_batched=_arr(ct.c_int32, slot_batched),
+        slot_leaf=_arr(ct.c_int64, slot_leaf), lss=_arr(ct.c_int64, lss or [0]),
+        step_slots=_arr(ct.c_int32, step_slots), kc=_arr(ct.c_int32, kcs), tile=_arr(ct.c_int32, tls),
+        nmodes=_arr(ct.c_int32, nmodes), modes=_arr(ct.c_int64, modes))
+    d = PlanDesc(len(slots), keep["slot_elems"], keep["slot_batched"], keep["slot_leaf"], n_sliced, keep["lss"],
+                 group_bits, len(steps), keep["step_slots"], keep["kc"], keep["tile"], keep["nmodes"],
+                 keep["modes"], out_slot)
+    p = ct.c_void_p()
+    ctx.check(ctx.lib.pq_plan_create(ctx.ptr, ct.cast(ct.pointer(d), ct.c_void_p), ct.byref(p)))
+    return Plan(ctx, p, slots, steps, out_slot, n_sliced, group_bits)

@@ -255,3 +255,111 @@ def plan_ledger(network, plan):
         steps.append((math.prod(A[1]), math.prod(B[1]), math.prod(C[1]), K))
     macs = sum(c * k for _, _, c, k in steps)
     return {"steps": steps, "macs": macs, "max_size": max((s[2] for s in steps), default=0)}
+
+
+# --------------------------------------------------------------------------- #
+# Slicing choice
+# --------------------------------------------------------------------------- #
+
+
+def _plan_masks(network, plan):
+    """Per step: bitmasks (over index ids) of C's indices, the summed indices
+    and every index in the step's subtree, plus log2 extents per index."""
+    ids = {}
+    ext = {}
+
+    def iid(x):
+        if x not in ids:
+            ids[x] = len(ids)
+        return ids[x]
+
+    tens = {}
+    for k, v in network.nodes.items():
+        mask = 0
+        for x, d in zip(v.indices, v.dims):
+            mask |= 1 << iid(x)
+            ext[ids[x]] = d
+        tens[k] = (mask, mask)        # (open indices, subtree indices)
+    counter = network.counters["node"]
+    steps = []
+    for a, b in plan:
+        (ma, sa), (mb, sb) = tens.pop(a), tens.pop(b)
+        k = ma & mb
+        c = ma ^ mb
+        counter += 1
+        tens[f"node_{counter}"] = (c, sa | sb)
+        steps.append((c, k, sa | sb))
+    return steps, ids, ext
+
+
+def _mask_size(mask, ext):
+    s = 1
+    while mask:
+        low = mask & -mask
+        s *= ext[low.bit_length() - 1]
+        mask ^= low
+    return s
+
+
+def sliced_cost(steps, ext, smask, nslices):
+    """Total MACs over all slices (independent steps once) and the largest
+    per-slice intermediate."""
+    total = 0
+    peak = 0
+    for c, k, sub in steps:
+        size_c = _mask_size(c & ~smask, ext)
+        size_k = _mask_size(k & ~smask, ext)
+        mult = nslices if (sub & smask) else 1
+        total += size_c * size_k * mult
+        peak = max(peak, size_c)
+    return total, peak
+
+
+def choose_slices(network, plan, n_bonds, max_elems=2 ** 28, exclude=(), candidates=200):
+    """Greedy choice of `n_bonds` extent-2 bonds to slice: each round adds the
+    bond that minimises the total sliced MACs (steps that do not depend on a
+    sliced bond run once, the others once per slice), preferring choices that
+    meet the per-slice size cap.  Returns (bonds, total_macs, peak)."""
+    steps, ids, ext = _plan_masks(network, plan)
+    inv = {v: k for k, v in ids.items()}
+    open_out = set()
+    for k, e in network.edges.items():
+        if e.src is None or e.dst is None:
+            open_out.add(k)
+    excl = set(exclude) | open_out
+    # candidate bonds: extent 2, internal, present in the costliest steps
+    order = sorted(range(len(steps)), key=lambda i: -_mask_size(steps[i][0], ext) * _mask_size(steps[i][1], ext))
+    cand = []
+    seen = set()
+    for i in order:
+        c, k, _ = steps[i]
+        m = c | k
+        while m:
+            low = m & -m
+            j = low.bit_length() - 1
+            m ^= low
+            if j in seen:
+                continue
+            seen.add(j)
+            if ext[j] == 2 and inv[j] not in excl:
+                cand.append(j)
+        if len(cand) >= candidates:
+            break
+    chosen = []
+    smask = 0
+    for r in range(n_bonds):
+        best = None
+        ns = 2 ** (r + 1)
+        for j in cand:
+            if smask >> j & 1:
+                continue
+            tot, peak = sliced_cost(steps, ext, smask | (1 << j), ns)
+            key = (peak > max_elems, tot, peak)
+            if best is None or key < best[0]:
+                best = (key, j)
+        if best is None:
+            raise ValueError("not enough bonds to slice")
+        chosen.append(best[1])
+        smask |= 1 << best[1]
+    tot, peak = sliced_cost(steps, ext, smask, 2 ** n_bonds)
+    return [inv[j] for j in chosen], tot, peak

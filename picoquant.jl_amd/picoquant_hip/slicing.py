@@ -71,3 +71,38 @@ def slice_tensor_network(network, bond_labels, bond_values):
         n1, n2 = network.edges[b].src, network.edges[b].dst
         replace_with_view(network, n1, b, (v, v))
         replace_with_view(network, n2, b, (v, v))
+
+
+def slice_with_map(network, bond_labels, bond_values):
+    """slice_tensor_network with the same operations, also returning the map
+    from each replaced node to the view node that now stands for it (so a
+    pair plan made for the unsliced network can be translated)."""
+    label_map = {}
+
+    def current(lab):
+        while lab in label_map:
+            lab = label_map[lab]
+        return lab
+
+    for b, v in zip(bond_labels, bond_values):
+        n1, n2 = network.edges[b].src, network.edges[b].dst
+        for n in (n1, n2):
+            view = f"node_{network.counters['node'] + 1}"
+            replace_with_view(network, n, b, (v, v))
+            label_map[n] = view
+    originals = {k: current(k) for k in label_map}
+    return originals
+
+
+def translate_plan(plan, label_map, old_counter, new_counter):
+    """Rename a pair plan for a network whose leaves were replaced (label_map)
+    and whose node counter moved from old_counter to new_counter."""
+    shift = new_counter - old_counter
+
+    def tr(lab):
+        if lab in label_map:
+            return label_map[lab]
+        k = int(lab.rsplit("_", 1)[1])
+        return f"node_{k + shift}" if k > old_counter else lab
+
+    return [[tr(a), tr(b)] for a, b in plan]

@@ -1,0 +1,131 @@
+"""Planned executor (pq_plan_*): whole plans over all slices with the FP64
+slice sum, against brute-force state vectors, the committed config-2
+amplitudes and the reference driver path on the same backend.
+Tolerances: ComplexF64 1e-12 relative; ComplexF32 1e-5 relative."""
+import json
+
+import numpy as np
+import pytest
+
+from oracle import pq_oracle as O
+import picoquant_hip as P
+from picoquant_hip import circuits as C
+from picoquant_hip import planner as PL
+from picoquant_hip.executor import compile_plan
+
+pytestmark = pytest.mark.gpu
+
+RTOL = {np.complex128: 1e-12, np.complex64: 1e-5}
+
+
+@pytest.fixture(scope="module", params=[np.complex128, np.complex64], ids=["c128", "c64"])
+def dtype(request):
+    return request.param
+
+
+def grid_net(rows, cols, depth, seed, cfg, dtype, backend=None):
+    n = rows * cols
+    tn = P.network_from_gates(n, C.grid_circuit_gates(rows, cols, depth, seed),
+                              backend=backend or P.HIPBackend(dtype), decompose=True)
+    P.add_input(tn, "0" * n)
+    P.add_output(tn, cfg)
+    return tn
+
+
+def batch_amps(sv, cfg):
+    open_q = [i for i, ch in enumerate(cfg) if ch == "x"]
+    out = []
+    for j in range(2 ** len(open_q)):
+        bits = [int(ch) if ch != "x" else 0 for ch in cfg]
+        for t, q in enumerate(open_q):
+            bits[q] = (j >> t) & 1
+        out.append(sv[sum(b << q for q, b in enumerate(bits))])
+    return np.array(out)
+
+
+def rel(got, want):
+    """Worst relative error; results are column-major (first open qubit fastest)."""
+    g = np.asarray(got).reshape(-1, order="F")
+    return float(np.max(np.abs(g - np.asarray(want).reshape(-1, order="F")) / np.abs(want).reshape(-1, order="F")))
+
+
+@pytest.mark.parametrize("nslice,group_bits", [(0, 0), (4, 0), (4, 2), (4, 4), (6, 3)])
+def test_grid3x3_sliced(dtype, nslice, group_bits):
+    cfg = "1x0x11x01"
+    tn = grid_net(3, 3, 6, 21, cfg, dtype)
+    sv = O.statevector(9, O.grid_circuit_gates(3, 3, 6, 21))
+    open_q = [i + 1 for i, ch in enumerate(cfg) if ch == "x"]
+    plan = PL.sweep_plan(tn, PL.grid_site_order(3, 3, open_last=open_q))
+    sliced = PL.choose_slices(tn, plan, nslice)[0] if nslice else []
+    p = compile_plan(tn, plan, sliced, group_bits=group_bits)
+    want = batch_amps(sv, cfg)
+    for use_graph in (False, True, True, True):
+        p.execute(use_graph)
+        assert rel(p.result(), want) < RTOL[dtype]
+    p.close()
+
+
+def test_greedy_plan_qft(dtype):
+    """A bgreedy (tau -> 0) plan of QFT-7 from a basis state, all outputs open."""
+    n, x = 7, 45
+    tn = P.network_from_gates(n, C.qft_gates(n), backend=P.HIPBackend(dtype))
+    P.add_input(tn, "".join(str((x >> q) & 1) for q in range(n)))
+    plan, _, _ = PL.bgreedy_single(tn, 1.0, 0.0)
+    p = compile_plan(tn, plan)
+    p.execute(True)
+    assert rel(p.result(), O.qft_closed_form(n, x)) < RTOL[dtype]
+
+
+def test_grid5x5_d12_batch(golden_dir, dtype):
+    """Config 2 through the planned executor: the open-column batch of 32
+    amplitudes with 6 sliced bonds, replayed as a graph."""
+    g = json.load(open(golden_dir / "grid5x5_d12.json"))
+    cfg = list(g["batch_base"])
+    for q in g["open_qubits_1based"]:
+        cfg[q - 1] = "x"
+    cfg = "".join(cfg)
+    tn = grid_net(5, 5, 12, g["seed"], cfg, dtype)
+    plan = PL.sweep_plan(tn, PL.grid_site_order(5, 5, open_last=g["open_qubits_1based"]))
+    sliced, _, peak = PL.choose_slices(tn, plan, 6)
+    p = compile_plan(tn, plan, sliced, group_bits=3)
+    want = np.array([complex(a, b) for a, b in g["batch_amplitudes"]])
+    for _ in range(3):
+        p.execute(True)
+        assert rel(p.result(), want) < RTOL[dtype]
+    st = p.stats()
+    assert st["out_elems"] == 32 and st["arena_bytes"] > 0
+    ms = p.profile()
+    assert ms.shape == (len(plan),) and np.all(ms >= 0)
+
+
+def test_plan_matches_reference_driver_per_slice():
+    """Slice-sum of the reference drivers (slice_tensor_network +
+    contract_network! per slice on HIPBackend) equals the planned executor."""
+    cfg = "0x1x01"
+    be = P.HIPBackend(np.complex128)
+    tn = grid_net(2, 3, 5, 6, cfg, np.complex128, be)
+    open_q = [i + 1 for i, ch in enumerate(cfg) if ch == "x"]
+    plan = PL.sweep_plan(tn, PL.grid_site_order(2, 3, open_last=open_q))
+    sliced = PL.choose_slices(tn, plan, 3)[0]
+    p = compile_plan(tn, plan, sliced, group_bits=1)
+    p.execute(True)
+    got = p.result().reshape(-1, order="F")
+    total = 0
+    for s in range(8):
+        t2 = grid_net(2, 3, 5, 6, cfg, np.complex128)
+        P.slice_tensor_network(t2, sliced, [((s >> j) & 1) + 1 for j in range(3)])
+        # the slice views are new nodes: plan by bgreedy on this slice
+        plan2, _, _ = PL.bgreedy_single(t2, 1.0, 0.0)
+        P.contract_network(t2, plan2, "vector")
+        total = total + t2.backend.load_tensor_data("result")
+    assert rel(got, total) < 1e-12
+
+
+def test_plan_rejects_bad_descriptors():
+    from picoquant_hip import _lib
+    tn = grid_net(2, 2, 3, 1, "0x10", np.complex128)
+    plan = PL.sweep_plan(tn, PL.grid_site_order(2, 2, open_last=[2]))
+    with pytest.raises(ValueError):
+        compile_plan(tn, plan[:-1])                    # leaves two tensors
+    with pytest.raises(_lib.PQError):
+        compile_plan(tn, plan, group_bits=0, kc=[_lib.KC_GEMM] * len(plan))   # M, N < 16
