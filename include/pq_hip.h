@@ -110,8 +110,9 @@ int pq_profile_read(pq_ctx* ctx, int kclass, int64_t* launches, double* total_ms
  * when the shape allows it (PQ_ERR_UNSUPPORTED otherwise). */
 int pq_set_kernel_override(pq_ctx* ctx, int kclass);
 
-/* GEMM workgroup tile override (-1 = automatic): 0:32x32 1:64x32 2:32x64
- * 3:64x64 4:128x64 5:64x128 (complex128 uses at most 64x64). */
+/* GEMM workgroup tile override (-1 = automatic): 0:32x32 1:64x32 2:32x64 3:64x64
+ * 4:128x64 5:64x128 6:64x16 7:128x16 8:256x16 9:16x64 10:128x32 11:128x128
+ * (complex128 tiles are capped at 4096 elements and 128 rows or columns). */
 int pq_set_gemm_tile(pq_ctx* ctx, int tile);
 
 /* ------------------------------------------------------------------------
@@ -145,6 +146,10 @@ typedef struct {
   const int32_t* step_nmodes;       /* [n_steps * 3] mode counts of M, N, K */
   const int64_t* step_modes;        /* concatenated (extent, s0, s1) triples */
   int32_t out_slot;                 /* per-slice result slot */
+  const int32_t* slot_dtype;        /* [n_slots] PQ_C128 / PQ_C64 (NULL = context type);
+                                       leaves must have the context's type */
+  const int32_t* step_kind;         /* [n_steps] 0 = contract A x B -> C, 1 = convert A -> C
+                                       (element type change, same layout; NULL = all contract) */
 } pq_plan_desc;
 
 int pq_plan_create(pq_ctx* ctx, const pq_plan_desc* desc, pq_plan** out);

@@ -12,9 +12,9 @@
 // complex64  -> v_mfma_f32_16x16x4_f32 (fp32 accumulate)
 // complex128 -> v_mfma_f64_16x16x4_f64 (fp64 accumulate)
 //
-// Tile: 256 lanes = 4 waves in a 2x2 arrangement; workgroup tile BM x BN,
-// each wave (BM/2) x (BN/2) as a grid of 16x16 MFMA tiles; BK = 16 per LDS
-// stage.  Lane layout of a 16x16x4 MFMA (64 lanes): A operand lane l holds
+// Tile: 256 lanes = 4 waves arranged WGM x WGN (2x2, 4x1 or 1x4); workgroup
+// tile BM x BN, each wave (BM/WGM) x (BN/WGN) as a grid of 16x16 MFMA tiles;
+// BK = 16 per LDS stage.  Extents below the tile are zero-padded.  Lane layout of a 16x16x4 MFMA (64 lanes): A operand lane l holds
 // A[l%16][l/16], B operand B[l/16][l%16]; accumulator register r holds
 // D[4*(l/16)+r][l%16] for f32 and D[l/16+4r][l%16] for f64.
 #include "pq_internal.h"
@@ -45,12 +45,14 @@ template <> struct mfma_t<double> {
 constexpr int BK = 16;
 constexpr int KTAB_MAX = 2048;   // K tables held in LDS
 
-template <typename C, int BM, int BN, bool AK, bool BK_FAST>
+template <typename C, int BM, int BN, int WGM, bool AK, bool BK_FAST>
 __global__ __launch_bounds__(256) void k_gemm(const C* __restrict__ A, const C* __restrict__ B,
                                               C* __restrict__ Out, GemmArgs g) {
   typedef typename real_of<C>::type R;
   typedef mfma_t<R> MF;
-  constexpr int WM = BM / 2, WN = BN / 2;        // wave tile
+  constexpr int WGN = 4 / WGM;
+  constexpr int WM = BM / WGM, WN = BN / WGN;    // wave tile
+  static_assert(WM % 16 == 0 && WN % 16 == 0, "wave tile must be a multiple of 16");
   constexpr int TM = WM / 16, TN = WN / 16;      // 16x16 tiles per wave
 
   __shared__ R As_r[BK][BM + 4], As_i[BK][BM + 4];
@@ -90,7 +92,7 @@ __global__ __launch_bounds__(256) void k_gemm(const C* __restrict__ A, const C* 
   __syncthreads();
 
   const int wave = tid / 64, lane = tid % 64;
-  const int wm = (wave / 2) * WM, wn = (wave % 2) * WN;
+  const int wm = (wave / WGN) * WM, wn = (wave % WGN) * WN;
   const int li = lane % 16, lk = lane / 16;
 
   typename MF::acc accr[TM][TN], acci[TM][TN];
@@ -170,27 +172,29 @@ __global__ __launch_bounds__(256) void k_gemm(const C* __restrict__ A, const C* 
       }
 }
 
-// Static LDS stays within 64 KiB: complex128 tiles up to 64x64.
+// Static LDS stays within 64 KiB: complex128 tiles up to 128 rows or
+// columns and 4096 elements.
 template <typename C, int BM, int BN>
 constexpr bool tile_fits() {
-  return sizeof(C) == 8 || BM * BN <= 64 * 64;
+  return sizeof(C) == 8 || (BM * BN <= 64 * 64 && BM <= 128 && BN <= 128);
 }
 
-template <typename C, int BM, int BN>
+template <typename C, int BM, int BN, int WGM>
 hipError_t launch_tile(const C* A, const C* B, C* Out, const GemmArgs& g, hipStream_t s) {
   if constexpr (!tile_fits<C, BM, BN>()) {
-    return launch_tile<C, 64, 64>(A, B, Out, g, s);
+    return launch_tile<C, (BM > 64 ? 64 : BM), (BN > 64 ? 64 : BN), WGM>(A, B, Out, g, s);
+  } else {
+    dim3 grid((g.M.total + BM - 1) / BM, (g.N.total + BN - 1) / BN, g.L.total);
+    if (g.a_kfast && g.b_kfast)
+      hipLaunchKernelGGL((k_gemm<C, BM, BN, WGM, true, true>), grid, dim3(256), 0, s, A, B, Out, g);
+    else if (g.a_kfast)
+      hipLaunchKernelGGL((k_gemm<C, BM, BN, WGM, true, false>), grid, dim3(256), 0, s, A, B, Out, g);
+    else if (g.b_kfast)
+      hipLaunchKernelGGL((k_gemm<C, BM, BN, WGM, false, true>), grid, dim3(256), 0, s, A, B, Out, g);
+    else
+      hipLaunchKernelGGL((k_gemm<C, BM, BN, WGM, false, false>), grid, dim3(256), 0, s, A, B, Out, g);
+    return hipGetLastError();
   }
-  dim3 grid((g.M.total + BM - 1) / BM, (g.N.total + BN - 1) / BN, g.L.total);
-  if (g.a_kfast && g.b_kfast)
-    hipLaunchKernelGGL((k_gemm<C, BM, BN, true, true>), grid, dim3(256), 0, s, A, B, Out, g);
-  else if (g.a_kfast)
-    hipLaunchKernelGGL((k_gemm<C, BM, BN, true, false>), grid, dim3(256), 0, s, A, B, Out, g);
-  else if (g.b_kfast)
-    hipLaunchKernelGGL((k_gemm<C, BM, BN, false, true>), grid, dim3(256), 0, s, A, B, Out, g);
-  else
-    hipLaunchKernelGGL((k_gemm<C, BM, BN, false, false>), grid, dim3(256), 0, s, A, B, Out, g);
-  return hipGetLastError();
 }
 
 }  // namespace
@@ -201,12 +205,18 @@ template <typename C>
 hipError_t launch_gemm(const C* A, const C* B, C* Out, const GemmArgs& g, hipStream_t s) {
   if (g.K.total > (uint32_t)KTAB_MAX || g.L.total < 1 || g.L.total > 65535) return hipErrorInvalidValue;
   switch (g.tile) {
-    case 0: return launch_tile<C, 32, 32>(A, B, Out, g, s);
-    case 1: return launch_tile<C, 64, 32>(A, B, Out, g, s);
-    case 2: return launch_tile<C, 32, 64>(A, B, Out, g, s);
-    case 3: return launch_tile<C, 64, 64>(A, B, Out, g, s);
-    case 4: return launch_tile<C, 128, 64>(A, B, Out, g, s);
-    case 5: return launch_tile<C, 64, 128>(A, B, Out, g, s);
+    case 0: return launch_tile<C, 32, 32, 2>(A, B, Out, g, s);
+    case 1: return launch_tile<C, 64, 32, 2>(A, B, Out, g, s);
+    case 2: return launch_tile<C, 32, 64, 2>(A, B, Out, g, s);
+    case 3: return launch_tile<C, 64, 64, 2>(A, B, Out, g, s);
+    case 4: return launch_tile<C, 128, 64, 2>(A, B, Out, g, s);
+    case 5: return launch_tile<C, 64, 128, 2>(A, B, Out, g, s);
+    case 6: return launch_tile<C, 64, 16, 4>(A, B, Out, g, s);
+    case 7: return launch_tile<C, 128, 16, 4>(A, B, Out, g, s);
+    case 8: return launch_tile<C, 256, 16, 4>(A, B, Out, g, s);
+    case 9: return launch_tile<C, 16, 64, 1>(A, B, Out, g, s);
+    case 10: return launch_tile<C, 128, 32, 4>(A, B, Out, g, s);
+    case 11: return launch_tile<C, 128, 128, 2>(A, B, Out, g, s);
     default: return hipErrorInvalidValue;
   }
 }

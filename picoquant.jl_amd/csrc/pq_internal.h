@@ -40,6 +40,25 @@ struct GemmArgs {
   int32_t tile;       // 0:32x32 1:64x32 2:32x64 3:64x64 4:128x64 5:64x128
 };
 
+// One problem of a grouped launch of small contractions (PQ_KC_BATCHED):
+// many independent steps of one dependency level in a single kernel launch,
+// one problem per grid row.  Pointers are for slice group 0; hA/hB/hC add
+// the leaf offset of each slice-group bit (group index passed at launch).
+#define PQ_MAX_HIGH_BITS 24
+struct MultiDesc {
+  const void* A;
+  const void* B;
+  void* C;
+  Group M, N, K;      // as GenArgs
+  Batch L;            // batch entries of one slice group (grid z)
+  int32_t nhigh;
+  int64_t hA[PQ_MAX_HIGH_BITS], hB[PQ_MAX_HIGH_BITS], hC[PQ_MAX_HIGH_BITS];
+};
+
+template <typename C>
+hipError_t launch_multi(const MultiDesc* descs, int n, unsigned blocks_x, unsigned batch, int64_t group,
+                        hipStream_t s);
+
 template <typename C>
 hipError_t launch_stream(const C* X, const C* Y, C* Out, const StreamArgs& a, hipStream_t s);
 template <typename C>

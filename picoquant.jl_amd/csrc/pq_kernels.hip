@@ -195,6 +195,41 @@ __global__ __launch_bounds__(256) void k_copy(const C* __restrict__ In, C* __res
   }
 }
 
+// ----------------------------------------------------------------- k_multi
+// Grouped small contractions: problem blockIdx.y, batch entry blockIdx.z,
+// one lane per output element (grid-stride over blockIdx.x), loop over K.
+template <typename C>
+__global__ __launch_bounds__(256) void k_multi(const MultiDesc* __restrict__ descs, int64_t group) {
+  const MultiDesc& d = descs[blockIdx.y];
+  if (blockIdx.z >= d.L.total) return;
+  int64_t oa, ob, oc;
+  batch_offsets(d.L, blockIdx.z, oa, ob, oc);
+  for (int j = 0; j < d.nhigh; ++j)
+    if ((group >> j) & 1) {
+      oa += d.hA[j];
+      ob += d.hB[j];
+      oc += d.hC[j];
+    }
+  const C* A = reinterpret_cast<const C*>(d.A) + oa;
+  const C* B = reinterpret_cast<const C*>(d.B) + ob;
+  C* Out = reinterpret_cast<C*>(d.C) + oc;
+  const uint32_t Mt = d.M.total, Nt = d.N.total, Kt = d.K.total;
+  const uint32_t total = Mt * Nt;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    uint32_t m = t % Mt, n = t / Mt;
+    int32_t pa, pc1, pb, pc2;
+    group_offsets(d.M, m, pa, pc1);
+    group_offsets(d.N, n, pb, pc2);
+    C acc = czero<C>();
+    for (uint32_t k = 0; k < Kt; ++k) {
+      int32_t ka, kb;
+      group_offsets(d.K, k, ka, kb);
+      cfma(acc, A[pa + ka], B[pb + kb]);
+    }
+    Out[pc1 + pc2] = acc;
+  }
+}
+
 inline unsigned grid_for(uint64_t work, unsigned block, unsigned cap = 8192) {
   uint64_t g = (work + block - 1) / block;
   if (g < 1) g = 1;
@@ -253,7 +288,16 @@ hipError_t launch_copy(const C* In, C* Out, const CopyArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename C>
+hipError_t launch_multi(const MultiDesc* descs, int n, unsigned blocks_x, unsigned batch, int64_t group,
+                        hipStream_t s) {
+  if (n < 1 || n > 65535 || batch < 1 || batch > 65535) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_multi<C>), dim3(blocks_x, n, batch), dim3(256), 0, s, descs, group);
+  return hipGetLastError();
+}
+
 #define PQ_INST(C)                                                                                   \
+  template hipError_t launch_multi<C>(const MultiDesc*, int, unsigned, unsigned, int64_t, hipStream_t); \
   template hipError_t launch_stream<C>(const C*, const C*, C*, const StreamArgs&, hipStream_t);      \
   template hipError_t launch_generic<C>(const C*, const C*, C*, const GenArgs&, hipStream_t);        \
   template hipError_t launch_reduce<C>(const C*, const C*, C*, const GenArgs&, C*, int, hipStream_t); \

@@ -3,16 +3,22 @@
 // network (src/layer2/slicing.jl), run as one stream-ordered program that
 // can be captured into a single hipGraph.
 //
-//  * Slots: every leaf and intermediate tensor.  Leaves are context tensors
-//    (holding their sliced bonds); intermediates live in one arena whose
-//    offsets come from a liveness pass at plan creation.
+//  * Slots: every leaf and intermediate tensor, each with its own element
+//    type (complex128 or complex64).  Leaves are context tensors (holding
+//    their sliced bonds); intermediates live in one arena whose offsets come
+//    from a liveness pass over the launch sequence.
+//  * Steps: contractions (A x B -> C, all three of one element type) and
+//    conversions (A -> C, same layout, element type change) -- so small
+//    steps can run in FP64 and the bulk in complex64.
 //  * Slices: each sliced bond has extent 2.  Slice s fixes bond j to bit j
 //    of s.  Steps whose operands do not depend on any sliced bond run once
 //    (phase 1); the others (phase 2) run once per slice group, batched over
 //    the 2^group_bits slices of the group on grid z, reading leaves through
 //    per-bond strides and batched intermediates through entry strides.
-//  * The per-slice result is summed over slices in FP64 (cross-slice
-//    accumulation), deterministic in slice order.
+//  * Small steps of one dependency level and element type share one grouped
+//    launch (the world lines of all qubits advance together).
+//  * The per-slice result is summed over slices in FP64, deterministic in
+//    slice order.
 #include "pq_rt.h"
 
 #include <algorithm>
@@ -42,21 +48,49 @@ __global__ void k_accum(const C* __restrict__ src, int64_t elems, int64_t stride
   }
 }
 
+// Element type conversion, n elements, same layout.
+template <typename S, typename D>
+__global__ void k_convert(const S* __restrict__ src, D* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    S v = src[i];
+    D w;
+    w.x = (typename real_of<D>::type)v.x;
+    w.y = (typename real_of<D>::type)v.y;
+    dst[i] = w;
+  }
+}
+
+size_t esize_of(int dtype) { return dtype == PQ_C128 ? 16 : 8; }
+
 struct Slot {
   int64_t elems = 0;      // per batch entry
   bool batched = false;
+  int dtype = PQ_C128;
   pq_tensor leaf = 0;
   std::vector<int64_t> slice_stride;  // leaves: per sliced bond
-  int64_t arena_off = -1;             // intermediates
+  int64_t arena_off = -1;             // intermediates (bytes)
   int producer = -1;
   int last_use = -1;
 };
 
 struct PStep {
+  int kind = 0;            // 0 contract, 1 convert
   int a, b, c;
   bool dep;
   StepSpec spec;           // resolved; batch strides for phase 2 steps
   double flops = 0, bytes = 0;
+  int level = 0;           // 1 + max level of the producers of A and B
+};
+
+// One entry of the launch sequence: a single step, or a grouped launch of
+// small steps of one dependency level (descriptors at desc_off).
+struct Launch {
+  bool grouped = false;
+  int step = -1;
+  std::vector<int> steps;  // grouped: member steps
+  int desc_off = 0;
+  int dtype = PQ_C128;
+  unsigned blocks_x = 1, batch = 1;
 };
 
 }  // namespace
@@ -65,6 +99,8 @@ struct pq_plan {
   std::vector<Slot> slots;
   std::vector<PStep> steps;
   std::vector<int> order1, order2;   // step indices of phase 1 / phase 2
+  std::vector<Launch> seq1, seq2;    // launch sequences of phase 1 / phase 2
+  MultiDesc* descs = nullptr;        // device copies of the grouped descriptors
   int n_sliced = 0, group_bits = 0;
   int64_t n_groups = 1;
   int out_slot = -1;
@@ -79,6 +115,7 @@ struct pq_plan {
   int64_t launches = 0;
   double alg_flops = 0, alg_bytes = 0;
   std::vector<hipEvent_t> ev;  // per-step event pairs for pq_plan_profile
+  std::vector<char> ev_rec;    // which pairs the last profiled run recorded
 };
 
 namespace {
@@ -101,33 +138,92 @@ int64_t span_of(const std::vector<Mode>& ms, bool first) {
   return s;
 }
 
+void free_plan(pq_plan* p) {
+  if (p->arena) hipFree(p->arena);
+  if (p->scratch) hipFree(p->scratch);
+  if (p->out_dev) hipFree(p->out_dev);
+  if (p->descs) hipFree(p->descs);
+  delete p;
+}
+
 int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
   int64_t nb = p->out_elems;
   unsigned g = (unsigned)std::min<int64_t>((nb + 255) / 256, 1024);
   hipLaunchKernelGGL(k_fill_zero, dim3(std::max(1u, g)), dim3(256), 0, c->stream, p->out_dev, 2 * nb);
   if (hipGetLastError() != hipSuccess) return fail(c, PQ_ERR_HIP, "fill kernel");
-  auto do_step = [&](int i, int64_t group) -> int {
-    PStep& st = p->steps[i];
-    if (profile_steps) hipEventRecord(p->ev[2 * i], c->stream);
-    int r = launch_step(c, slot_ptr(c, p, st.a, group), slot_ptr(c, p, st.b, group), slot_ptr(c, p, st.c, group),
+  const int G = 1 << p->group_bits;
+  auto do_launch = [&](const Launch& L, int64_t group) -> int {
+    if (!L.grouped) {
+      const int i = L.step;
+      PStep& st = p->steps[i];
+      if (profile_steps) hipEventRecord(p->ev[2 * i], c->stream);
+      int r = PQ_OK;
+      if (st.kind == 1) {
+        const Slot& S = p->slots[st.a];
+        const Slot& D = p->slots[st.c];
+        int64_t n = D.elems * (D.batched ? G : 1);
+        unsigned gc = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+        const void* src = slot_ptr(c, p, st.a, group);
+        void* dst = slot_ptr(c, p, st.c, group);
+        ProfScope ps(c, PQ_KC_COPY, (double)n * (esize_of(S.dtype) + esize_of(D.dtype)), 0);
+        if (S.dtype == PQ_C128 && D.dtype == PQ_C64)
+          hipLaunchKernelGGL((k_convert<c128, c64>), dim3(std::max(1u, gc)), dim3(256), 0, c->stream,
+                             (const c128*)src, (c64*)dst, n);
+        else if (S.dtype == PQ_C64 && D.dtype == PQ_C128)
+          hipLaunchKernelGGL((k_convert<c64, c128>), dim3(std::max(1u, gc)), dim3(256), 0, c->stream,
+                             (const c64*)src, (c128*)dst, n);
+        else if (S.dtype == PQ_C128)
+          hipLaunchKernelGGL((k_convert<c128, c128>), dim3(std::max(1u, gc)), dim3(256), 0, c->stream,
+                             (const c128*)src, (c128*)dst, n);
+        else
+          hipLaunchKernelGGL((k_convert<c64, c64>), dim3(std::max(1u, gc)), dim3(256), 0, c->stream,
+                             (const c64*)src, (c64*)dst, n);
+        if (hipGetLastError() != hipSuccess) r = fail(c, PQ_ERR_HIP, "convert kernel");
+      } else {
+        r = launch_step(c, slot_ptr(c, p, st.a, group), slot_ptr(c, p, st.b, group), slot_ptr(c, p, st.c, group),
                         st.spec, p->scratch);
-    if (profile_steps) hipEventRecord(p->ev[2 * i + 1], c->stream);
-    return r;
+      }
+      if (profile_steps) {
+        hipEventRecord(p->ev[2 * i + 1], c->stream);
+        p->ev_rec[i] = 1;
+      }
+      return r;
+    }
+    // Grouped: the events of the first member step bracket the whole launch.
+    if (profile_steps) hipEventRecord(p->ev[2 * L.steps[0]], c->stream);
+    double fl = 0, by = 0;
+    for (int i : L.steps) {
+      fl += p->steps[i].flops;
+      by += p->steps[i].bytes;
+    }
+    hipError_t e;
+    {
+      ProfScope ps(c, PQ_KC_BATCHED, by, fl);
+      e = (L.dtype == PQ_C128)
+              ? launch_multi<c128>(p->descs + L.desc_off, (int)L.steps.size(), L.blocks_x, L.batch, group, c->stream)
+              : launch_multi<c64>(p->descs + L.desc_off, (int)L.steps.size(), L.blocks_x, L.batch, group, c->stream);
+    }
+    if (profile_steps) {
+      hipEventRecord(p->ev[2 * L.steps[0] + 1], c->stream);
+      p->ev_rec[L.steps[0]] = 1;
+    }
+    if (e != hipSuccess) return hipfail(c, e, "grouped kernel");
+    return PQ_OK;
   };
-  for (int i : p->order1) {
-    int r = do_step(i, 0);
+  for (const Launch& L : p->seq1) {
+    int r = do_launch(L, 0);
     if (r) return r;
   }
   const Slot& os = p->slots[p->out_slot];
-  int nbatch = os.batched ? (1 << p->group_bits) : 1;
+  int nbatch = os.batched ? G : 1;
   for (int64_t grp = 0; grp < p->n_groups; ++grp) {
-    for (int i : p->order2) {
-      int r = do_step(i, grp);
+    for (const Launch& L : p->seq2) {
+      int r = do_launch(L, grp);
       if (r) return r;
     }
     const void* src = slot_ptr(c, p, p->out_slot, grp);
     unsigned ga = (unsigned)std::min<int64_t>((os.elems + 255) / 256, 1024);
-    if (c->dtype == PQ_C128)
+    if (os.dtype == PQ_C128)
       hipLaunchKernelGGL((k_accum<c128>), dim3(std::max(1u, ga)), dim3(256), 0, c->stream, (const c128*)src,
                          os.elems, os.elems, nbatch, p->out_dev);
     else
@@ -154,7 +250,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
   p->out_slot = d->out_slot;
   const int G = 1 << d->group_bits;
   auto bad = [&](const std::string& m) {
-    delete p;
+    free_plan(p);
     return fail(c, PQ_ERR_ARG, m);
   };
   p->slots.resize(d->n_slots);
@@ -163,11 +259,14 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     sl.elems = d->slot_elems[s];
     sl.batched = d->slot_batched[s] != 0;
     sl.leaf = d->slot_leaf[s];
+    sl.dtype = d->slot_dtype ? d->slot_dtype[s] : c->dtype;
+    if (sl.dtype != PQ_C128 && sl.dtype != PQ_C64) return bad("bad slot element type");
     if (sl.elems < 1 || sl.elems >= (1ll << 31)) return bad("slot size out of range");
     if (sl.leaf) {
       Tensor* t = get(c, sl.leaf);
       if (!t) return bad("unknown leaf tensor");
       if (sl.batched) return bad("a leaf cannot be batched");
+      if (sl.dtype != c->dtype) return bad("a leaf must have the context's element type");
       int64_t span = 0;
       for (int i = 0; i < t->rank; ++i) span += (t->dims[i] - 1) * t->strides[i];
       if (span + 1 > (int64_t)(t->buf->bytes / c->esize) - t->off) return bad("leaf view exceeds its buffer");
@@ -181,14 +280,19 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
   p->steps.resize(d->n_steps);
   for (int i = 0; i < d->n_steps; ++i) {
     PStep& st = p->steps[i];
+    st.kind = d->step_kind ? d->step_kind[i] : 0;
+    if (st.kind != 0 && st.kind != 1) return bad("bad step kind");
     st.a = d->step_slots[3 * i];
     st.b = d->step_slots[3 * i + 1];
     st.c = d->step_slots[3 * i + 2];
+    if (st.kind == 1) st.b = st.a;
     for (int x : {st.a, st.b, st.c})
       if (x < 0 || x >= d->n_slots) return bad("step slot out of range");
     if (p->slots[st.c].leaf || produced[st.c] >= 0) return bad("step output must be a fresh intermediate");
     for (int x : {st.a, st.b})
       if (!p->slots[x].leaf && produced[x] < 0) return bad("step reads a slot before it is produced");
+    st.level = 1 + std::max(p->slots[st.a].leaf ? 0 : p->steps[p->slots[st.a].producer].level,
+                            p->slots[st.b].leaf ? 0 : p->steps[p->slots[st.b].producer].level);
     produced[st.c] = i;
     p->slots[st.c].producer = i;
     p->slots[st.a].last_use = i;
@@ -206,9 +310,21 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     for (auto* v : {&st.spec.M, &st.spec.N, &st.spec.K})
       for (auto& m : *v)
         if (m.ext < 1 || m.s0 < 0 || m.s1 < 0) return bad("bad mode");
+    const Slot &A = p->slots[st.a], &B = p->slots[st.b], &Cs = p->slots[st.c];
+    st.dep = Cs.batched;
+    if (!st.dep && (A.batched || B.batched)) return bad("unbatched output from a batched operand");
+    if (st.kind == 1) {
+      if (A.leaf || A.elems != Cs.elems || A.batched != Cs.batched) return bad("convert needs a same-shaped intermediate");
+      const double n = (double)Cs.elems * (Cs.batched ? (double)G * p->n_groups : 1.0);
+      st.bytes = n * (esize_of(A.dtype) + esize_of(Cs.dtype));
+      p->alg_bytes += st.bytes;
+      (st.dep ? p->order2 : p->order1).push_back(i);
+      continue;
+    }
+    if (A.dtype != Cs.dtype || B.dtype != Cs.dtype) return bad("contraction operands differ in element type");
+    st.spec.dtype = Cs.dtype;
     // In-bounds check of every operand for one batch entry; a leaf must also
     // hold every slice offset on top of the access span.
-    const Slot &A = p->slots[st.a], &B = p->slots[st.b], &Cs = p->slots[st.c];
     auto fits = [&](const Slot& sl, int64_t span) -> bool {
       if (!sl.leaf) return span < sl.elems;
       Tensor* t = get(c, sl.leaf);
@@ -220,8 +336,6 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     if (!fits(A, span_of(st.spec.M, true) + span_of(st.spec.K, true))) return bad("A access out of range");
     if (!fits(B, span_of(st.spec.N, true) + span_of(st.spec.K, false))) return bad("B access out of range");
     if (!fits(Cs, span_of(st.spec.M, false) + span_of(st.spec.N, false))) return bad("C access out of range");
-    st.dep = Cs.batched;
-    if (!st.dep && (A.batched || B.batched)) return bad("unbatched output from a batched operand");
     // Batch group over the slices of a group.
     st.spec.L = single_batch();
     if (st.dep && G > 1) {
@@ -241,19 +355,20 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     }
     st.spec.kc = d->step_kc ? d->step_kc[i] : -1;
     st.spec.tile = d->step_tile ? d->step_tile[i] : -1;
-    int r = resolve_step(c, st.spec, c->esize);
+    int r = resolve_step(c, st.spec, esize_of(st.spec.dtype));
     if (r) {
       std::string msg = c->err;
-      delete p;
+      free_plan(p);
       return fail(c, r, "step " + std::to_string(i) + ": " + msg);
     }
     const double nbat = st.dep ? (double)G * p->n_groups : 1.0;
+    const double es = (double)esize_of(st.spec.dtype);
     st.flops = 8.0 * st.spec.Mt * st.spec.Nt * st.spec.Kt * nbat;
-    st.bytes = (double)c->esize * (st.spec.Mt * st.spec.Kt + st.spec.Kt * st.spec.Nt + st.spec.Mt * st.spec.Nt) * nbat;
+    st.bytes = es * (st.spec.Mt * st.spec.Kt + st.spec.Kt * st.spec.Nt + st.spec.Mt * st.spec.Nt) * nbat;
     p->alg_flops += st.flops;
     p->alg_bytes += st.bytes;
     (st.dep ? p->order2 : p->order1).push_back(i);
-    p->scratch_bytes = std::max(p->scratch_bytes, step_scratch_bytes(st.spec, c->esize));
+    p->scratch_bytes = std::max(p->scratch_bytes, step_scratch_bytes(st.spec, esize_of(st.spec.dtype)));
   }
   if (!p->slots[p->out_slot].leaf && produced[p->out_slot] < 0) return bad("output slot never produced");
   if (d->n_sliced > 0 && !p->slots[p->out_slot].batched) return bad("sliced plan with a slice-independent output");
@@ -270,13 +385,69 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     for (int i = 0; i < t->rank; ++i) span += (t->dims[i] - 1) * t->strides[i];
     if (extra > span) return bad("slice strides exceed the leaf");
   }
-  // Liveness over the sequence [phase 1][phase 2 of one group]; anything a
-  // phase-2 step reads stays live to the end (every group reads it again).
+  // Launch sequences: by dependency level; the small contraction steps of a
+  // level and element type share one grouped launch, the others launch alone.
+  auto small = [&](const PStep& st) {
+    return st.kind == 0 && (st.spec.kc == PQ_KC_STREAM || st.spec.kc == PQ_KC_GENERIC) &&
+           st.spec.Mt * st.spec.Nt <= 65536 && st.spec.Kt <= 64;
+  };
+  auto build_seq = [&](const std::vector<int>& order, std::vector<Launch>& seq) {
+    std::vector<int> o = order;
+    std::stable_sort(o.begin(), o.end(), [&](int x, int y) { return p->steps[x].level < p->steps[y].level; });
+    size_t i = 0;
+    while (i < o.size()) {
+      const int lvl = p->steps[o[i]].level;
+      std::vector<int> grp[2];
+      size_t j = i;
+      for (; j < o.size() && p->steps[o[j]].level == lvl; ++j) {
+        const PStep& st = p->steps[o[j]];
+        const int ti = st.spec.dtype == PQ_C128 ? 0 : 1;
+        if (small(st) && (int)grp[ti].size() < 65535) {
+          grp[ti].push_back(o[j]);
+        } else {
+          Launch L;
+          L.step = o[j];
+          seq.push_back(L);
+        }
+      }
+      for (int ti = 0; ti < 2; ++ti) {
+        if (grp[ti].size() == 1) {
+          Launch L;
+          L.step = grp[ti][0];
+          seq.push_back(L);
+        } else if (grp[ti].size() > 1) {
+          Launch L;
+          L.grouped = true;
+          L.steps = grp[ti];
+          L.dtype = ti == 0 ? PQ_C128 : PQ_C64;
+          seq.push_back(L);
+        }
+      }
+      i = j;
+    }
+  };
+  build_seq(p->order1, p->seq1);
+  build_seq(p->order2, p->seq2);
+  // Liveness over the launch sequence [phase 1][phase 2 of one group]; the
+  // steps of one launch share a position; anything a phase-2 step reads
+  // stays live to the end (every group reads it again).
   std::vector<int> pos(d->n_steps);
   int t = 0;
-  for (int i : p->order1) pos[i] = t++;
+  for (const Launch& L : p->seq1) {
+    if (L.grouped)
+      for (int i : L.steps) pos[i] = t;
+    else
+      pos[L.step] = t;
+    ++t;
+  }
   const int phase2_start = t;
-  for (int i : p->order2) pos[i] = t++;
+  for (const Launch& L : p->seq2) {
+    if (L.grouped)
+      for (int i : L.steps) pos[i] = t;
+    else
+      pos[L.step] = t;
+    ++t;
+  }
   const int end = t;
   struct Live { int s, start, stop; size_t bytes; };
   std::vector<Live> lives;
@@ -291,13 +462,12 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     for (int i : p->order2)
       if (p->steps[i].a == s || p->steps[i].b == s) read_in_2 = true;
     if (read_in_2 && start < phase2_start) stop = end;
-    size_t bytes = (size_t)sl.elems * c->esize * (sl.batched ? G : 1);
+    size_t bytes = (size_t)sl.elems * esize_of(sl.dtype) * (sl.batched ? G : 1);
     bytes = (bytes + 255) / 256 * 256;
     lives.push_back({s, start, stop, bytes});
   }
   // First-fit over time-ordered intervals.
   std::sort(lives.begin(), lives.end(), [](const Live& x, const Live& y) { return x.start < y.start; });
-  std::vector<std::pair<size_t, size_t>> busy;  // (off, bytes) of live blocks
   std::vector<Live> active;
   size_t top = 0;
   for (auto& L : lives) {
@@ -316,26 +486,67 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     active.push_back(L);
   }
   p->arena_bytes = std::max<size_t>(top, 256);
-  if (hipMalloc(&p->arena, p->arena_bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    delete p;
-    return fail(c, PQ_ERR_OOM, "plan arena allocation failed");
-  }
-  if (p->scratch_bytes && hipMalloc(&p->scratch, p->scratch_bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    hipFree(p->arena);
-    delete p;
-    return fail(c, PQ_ERR_OOM, "plan scratch allocation failed");
-  }
   p->out_elems = p->slots[p->out_slot].elems;
-  if (hipMalloc((void**)&p->out_dev, sizeof(double) * 2 * p->out_elems) != hipSuccess) {
+  if (hipMalloc(&p->arena, p->arena_bytes) != hipSuccess ||
+      (p->scratch_bytes && hipMalloc(&p->scratch, p->scratch_bytes) != hipSuccess) ||
+      hipMalloc((void**)&p->out_dev, sizeof(double) * 2 * p->out_elems) != hipSuccess) {
     (void)hipGetLastError();
-    hipFree(p->arena);
-    if (p->scratch) hipFree(p->scratch);
-    delete p;
-    return fail(c, PQ_ERR_OOM, "plan output allocation failed");
+    free_plan(p);
+    return fail(c, PQ_ERR_OOM, "plan memory allocation failed");
   }
-  p->launches = 1 + (int64_t)p->order1.size() + p->n_groups * ((int64_t)p->order2.size() + 1);
+  // Grouped descriptors (pointers of slice group 0 + per-bit leaf offsets).
+  if (d->n_sliced - d->group_bits > PQ_MAX_HIGH_BITS) {
+    bool any = false;
+    for (auto* seq : {&p->seq1, &p->seq2})
+      for (const Launch& L : *seq) any = any || L.grouped;
+    if (any) {
+      free_plan(p);
+      return fail(c, PQ_ERR_UNSUPPORTED, "too many slice groups for grouped launches");
+    }
+  }
+  std::vector<MultiDesc> hd;
+  for (auto* seq : {&p->seq1, &p->seq2}) {
+    for (Launch& L : *seq) {
+      if (!L.grouped) continue;
+      L.desc_off = (int)hd.size();
+      uint64_t maxw = 1;
+      unsigned maxb = 1;
+      for (int i : L.steps) {
+        const PStep& st = p->steps[i];
+        MultiDesc m;
+        memset(&m, 0, sizeof(m));
+        m.A = slot_ptr(c, p, st.a, 0);
+        m.B = slot_ptr(c, p, st.b, 0);
+        m.C = slot_ptr(c, p, st.c, 0);
+        if (!fill_group(m.M, st.spec.M) || !fill_group(m.N, st.spec.N) || !fill_group(m.K, st.spec.K)) {
+          free_plan(p);
+          return fail(c, PQ_ERR_UNSUPPORTED, "too many modes in a grouped step");
+        }
+        m.L = st.spec.L;
+        m.nhigh = d->n_sliced - d->group_bits;
+        for (int j = 0; j < m.nhigh; ++j) {
+          const int bit = d->group_bits + j;
+          m.hA[j] = p->slots[st.a].leaf ? p->slots[st.a].slice_stride[bit] : 0;
+          m.hB[j] = p->slots[st.b].leaf ? p->slots[st.b].slice_stride[bit] : 0;
+          m.hC[j] = 0;
+        }
+        hd.push_back(m);
+        maxw = std::max<uint64_t>(maxw, st.spec.Mt * st.spec.Nt);
+        maxb = std::max<unsigned>(maxb, st.spec.L.total);
+      }
+      L.blocks_x = (unsigned)std::min<uint64_t>((maxw + 255) / 256, 64);
+      L.batch = maxb;
+    }
+  }
+  if (!hd.empty()) {
+    if (hipMalloc((void**)&p->descs, hd.size() * sizeof(MultiDesc)) != hipSuccess ||
+        hipMemcpy(p->descs, hd.data(), hd.size() * sizeof(MultiDesc), hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipGetLastError();
+      free_plan(p);
+      return fail(c, PQ_ERR_OOM, "descriptor upload failed");
+    }
+  }
+  p->launches = 1 + (int64_t)p->seq1.size() + p->n_groups * ((int64_t)p->seq2.size() + 1);
   *out = p;
   return PQ_OK;
 }
@@ -388,7 +599,11 @@ int pq_plan_stats(const pq_plan* p, int64_t* arena_bytes, int64_t* launches, dou
 int pq_plan_step_info(const pq_plan* p, int step, int* kclass, int* tile, int* dep, double* flops, double* bytes) {
   if (!p || step < 0 || step >= (int)p->steps.size()) return PQ_ERR_ARG;
   const PStep& s = p->steps[step];
-  if (kclass) *kclass = s.spec.kc;
+  int kc = s.kind == 1 ? PQ_KC_COPY : s.spec.kc;
+  for (auto* seq : {&p->seq1, &p->seq2})
+    for (const Launch& L : *seq)
+      if (L.grouped && std::find(L.steps.begin(), L.steps.end(), step) != L.steps.end()) kc = PQ_KC_BATCHED;
+  if (kclass) *kclass = kc;
   if (tile) *tile = s.spec.tile;
   if (dep) *dep = s.dep ? 1 : 0;
   if (flops) *flops = s.flops;
@@ -403,27 +618,27 @@ int pq_plan_profile(pq_ctx* c, pq_plan* p, double* step_ms) {
     p->ev.resize(2 * n);
     for (auto& e : p->ev) hipEventCreate(&e);
   }
+  p->ev_rec.assign(n, 0);
   for (size_t i = 0; i < n; ++i) step_ms[i] = 0;
-  // Phase 2 steps run once per group: time them group by group.
   hipStreamSynchronize(c->stream);
-  int r = PQ_OK;
-  {
-    // phase 1 and the first group, with events around every step
-    std::vector<int> o2 = p->order2;
-    int64_t ng = p->n_groups;
-    p->n_groups = 1;
-    r = run_program(c, p, true);
-    p->n_groups = ng;
-    if (r) return r;
-    hipError_t e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) return hipfail(c, e, "profile sync");
-    for (size_t i = 0; i < n; ++i) {
-      float ms = 0;
-      if (hipEventElapsedTime(&ms, p->ev[2 * i], p->ev[2 * i + 1]) == hipSuccess)
-        step_ms[i] = p->steps[i].dep ? ms * (double)ng : ms;
-    }
+  // Phase 1 and one slice group with events around every launch; phase-2
+  // times are scaled by the number of groups.
+  int64_t ng = p->n_groups;
+  p->n_groups = 1;
+  int r = run_program(c, p, true);
+  p->n_groups = ng;
+  if (r) return r;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hipfail(c, e, "profile sync");
+  // Grouped launches are timed as a whole on their first member step.
+  for (size_t i = 0; i < n; ++i) {
+    if (!p->ev_rec[i]) continue;
+    float ms = 0;
+    e = hipEventElapsedTime(&ms, p->ev[2 * i], p->ev[2 * i + 1]);
+    if (e != hipSuccess) return hipfail(c, e, "profile event");
+    step_ms[i] = p->steps[i].dep ? ms * (double)ng : ms;
   }
-  return r;
+  return PQ_OK;
 }
 
 int pq_plan_destroy(pq_ctx* c, pq_plan* p) {
@@ -432,10 +647,7 @@ int pq_plan_destroy(pq_ctx* c, pq_plan* p) {
   if (p->exec) hipGraphExecDestroy(p->exec);
   if (p->graph) hipGraphDestroy(p->graph);
   for (auto& e : p->ev) hipEventDestroy(e);
-  if (p->arena) hipFree(p->arena);
-  if (p->scratch) hipFree(p->scratch);
-  if (p->out_dev) hipFree(p->out_dev);
-  delete p;
+  free_plan(p);
   return PQ_OK;
 }
 

@@ -46,6 +46,7 @@ struct StepSpec {
   int kc = -1;       // PQ_KC_* after resolve_step
   int tile = -1;     // GEMM tile after resolve_step
   bool stream_ab = true;  // stream kernel: X = A (true) or X = B
+  int dtype = -1;         // PQ_C128 / PQ_C64; -1 = the context's
   uint64_t Mt = 1, Nt = 1, Kt = 1;
 };
 

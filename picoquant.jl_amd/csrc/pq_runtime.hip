@@ -205,8 +205,10 @@ int materialize_into(pq_ctx* c, const Tensor& src, Tensor& dst) {
 // Default workgroup tile for the MFMA GEMM: big tiles when both free extents
 // fill them, narrow ones otherwise (complex128 is limited to 64x64).
 static int pick_tile(uint64_t Mt, uint64_t Nt, size_t esize) {
+  if (Nt <= 16) return Mt >= 4096 && esize == 8 ? 8 : (Mt >= 128 ? 7 : 6);
+  if (Mt <= 16) return 9;
   if (Mt <= 32 && Nt <= 32) return 0;
-  if (Nt <= 32) return 1;
+  if (Nt <= 32) return Mt >= 128 ? 10 : 1;
   if (Mt <= 32) return 2;
   if (esize == 8 && Mt >= 4096 && Nt >= 128) return 5;
   if (esize == 8 && Mt >= 4096 && Nt >= 64) return 4;
@@ -227,7 +229,10 @@ int resolve_step(pq_ctx* c, StepSpec& s, size_t esize) {
   for (auto& m : s.K) s.Kt *= m.ext;
   const uint64_t Mt = s.Mt, Nt = s.Nt, Kt = s.Kt;
   const bool ab = stream_fits(Kt, Nt, esize), ba = stream_fits(Kt, Mt, esize);
-  const bool gemm_ok = Kt <= (uint64_t)gemm_k_limit() && Mt >= 16 && Nt >= 16 && Kt >= 4;
+  // MFMA tiles zero-pad short extents: worth it once one free side fills a
+  // 16-wide tile and there is enough work per output.
+  const bool gemm_ok = Kt <= (uint64_t)gemm_k_limit() && Kt >= 4 && std::max(Mt, Nt) >= 16 &&
+                       Mt * Nt >= 128;
   int kc = s.kc >= 0 ? s.kc : c->override_kc;
   if (kc < 0) {
     if ((ab && Kt * Nt <= 64) || (ba && Kt * Mt <= 64)) kc = PQ_KC_STREAM;
@@ -317,7 +322,8 @@ static int launch_typed(pq_ctx* c, const C* pa, const C* pb, C* pc, const StepSp
 }
 
 int launch_step(pq_ctx* c, const void* A, const void* B, void* Out, const StepSpec& s, void* scratch) {
-  if (c->dtype == PQ_C128)
+  const int dt = s.dtype >= 0 ? s.dtype : c->dtype;
+  if (dt == PQ_C128)
     return launch_typed<c128>(c, (const c128*)A, (const c128*)B, (c128*)Out, s, scratch);
   return launch_typed<c64>(c, (const c64*)A, (const c64*)B, (c64*)Out, s, scratch);
 }
@@ -682,7 +688,7 @@ int pq_profile_read(pq_ctx* c, int kc, int64_t* launches, double* total_ms, doub
 }
 
 int pq_set_gemm_tile(pq_ctx* c, int tile) {
-  if (!c || tile < -1 || tile > 5) return PQ_ERR_ARG;
+  if (!c || tile < -1 || tile > 11) return PQ_ERR_ARG;
   c->gemm_tile = tile;
   return PQ_OK;
 }

@@ -38,6 +38,8 @@ class PlanDesc(ct.Structure):
         ("step_nmodes", ct.POINTER(ct.c_int32)),
         ("step_modes", ct.POINTER(ct.c_int64)),
         ("out_slot", ct.c_int32),
+        ("slot_dtype", ct.POINTER(ct.c_int32)),
+        ("step_kind", ct.POINTER(ct.c_int32)),
     ]
 
 
@@ -143,17 +145,62 @@ def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf
     return slots, steps, out_slot
 
 
+SMALL_ELEMS = 1 << 16   # steps up to this size run in FP64 under precision="mixed"
+
+
+def assign_precision(slots, steps, ctx_code, precision, small_elems=SMALL_ELEMS):
+    """Element type per slot and the step list with conversions:
+    [(kind, a, b, c, M, N, K)], kind 0 = contract, 1 = convert a -> c.
+
+    "uniform": every slot in the context's type.  "mixed" (context type
+    complex128): a step whose operands and result all have at most
+    small_elems elements per slice runs in FP64 while its operands are FP64;
+    every other step runs in complex64, after a conversion of FP64 operands.
+    Leaves stay FP64, so a complex64 step that reads a leaf runs in FP64
+    instead (its other operand converted up)."""
+    n0 = len(slots)
+    dt = [ctx_code] * n0
+    out = []
+    if precision == "uniform":
+        return dt, [(0, sa, sb, sc, M, N, K) for sa, sb, sc, M, N, K in steps], slots
+    if ctx_code != _lib.PQ_C128:
+        raise ValueError('precision="mixed" needs a ComplexF64 backend (FP64 leaves)')
+    slots = list(slots)
+    for sa, sb, sc, M, N, K in steps:
+        small = all(slots[x].elems <= small_elems for x in (sa, sb, sc))
+        want = _lib.PQ_C128 if (small and dt[sa] == _lib.PQ_C128 and dt[sb] == _lib.PQ_C128) else _lib.PQ_C64
+        if want == _lib.PQ_C64 and (slots[sa].leaf or slots[sb].leaf):
+            want = _lib.PQ_C128
+        ops = []
+        for x in (sa, sb):
+            if dt[x] != want:
+                src = slots[x]
+                y = SlotInfo()
+                y.idx, y.ext, y.stride = list(src.idx), dict(src.ext), dict(src.stride)
+                y.dep, y.elems = src.dep, src.elems
+                slots.append(y)
+                dt.append(want)
+                out.append((1, x, x, len(slots) - 1, [], [], []))
+                ops.append(len(slots) - 1)
+            else:
+                ops.append(x)
+        dt[sc] = want
+        out.append((0, ops[0], ops[1], sc, M, N, K))
+    return dt, out, slots
+
+
 class Plan:
     """A compiled plan on a context.  execute() enqueues one full run."""
 
-    def __init__(self, ctx, ptr, slots, steps, out_slot, n_sliced, group_bits):
+    def __init__(self, ctx, ptr, slots, steps, out_slot, n_sliced, group_bits, dtypes=None):
         self.ctx = ctx
         self.ptr = ptr
         self.slots = slots
-        self.steps = steps
+        self.steps = steps          # [(kind, a, b, c, M, N, K)]
         self.out_slot = out_slot
         self.n_sliced = n_sliced
         self.group_bits = group_bits
+        self.dtypes = dtypes
         out = slots[out_slot]
         self.out_shape = tuple(out.ext[x] for x in out.idx)
 
@@ -223,8 +270,13 @@ def choose_group_bits(slots, n_sliced, budget_bytes, esize):
 
 
 def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=None, tiles=None,
-                 budget_bytes=64 << 30):
-    """Compile a pair plan of a network whose backend is a HIPBackend."""
+                 budget_bytes=64 << 30, precision="uniform"):
+    """Compile a pair plan of a network whose backend is a HIPBackend.
+
+    precision: "uniform" (every step in the backend's type) or "mixed" (FP64
+    backend: small steps in FP64, large ones in complex64 on FP32 MFMA; see
+    assign_precision).  kc / tiles, when given, are per contraction step of
+    `plan` (kernel class and GEMM tile, -1 = automatic)."""
     backend = network.backend
     ctx = backend.ctx
     handles, strides = _leaf_layout(backend)
@@ -233,8 +285,9 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
         raise KeyError(f"leaf tensors missing on the device: {missing[:4]}")
     slots, steps, out_slot = lower_plan(network, plan, sliced, out_order, handles, strides)
     n_sliced = len(list(sliced))
-    esize = 16 if backend.dtype == np.complex128 else 8
+    dts, steps, slots = assign_precision(slots, steps, ctx.code, precision)
     if group_bits is None:
+        esize = 8 if precision == "mixed" or ctx.code == _lib.PQ_C64 else 16
         group_bits = choose_group_bits(slots, n_sliced, budget_bytes, esize)
     slot_elems = [s.elems for s in slots]
     slot_batched = [1 if (s.dep and not s.leaf) else 0 for s in slots]
@@ -242,23 +295,34 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
     lss = []
     for s in slots:
         lss += (s.slice_stride if s.leaf else [0] * n_sliced)
-    step_slots, nmodes, modes = [], [], []
-    for sa, sb, sc, M, N, K in steps:
+    step_slots, nmodes, modes, kinds = [], [], [], []
+    kcs_in = list(kc) if kc is not None else None
+    tls_in = list(tiles) if tiles is not None else None
+    kcs, tls = [], []
+    ci = 0
+    for kind, sa, sb, sc, M, N, K in steps:
         step_slots += [sa, sb, sc]
         nmodes += [len(M), len(N), len(K)]
+        kinds.append(kind)
         for grp in (M, N, K):
             for m in grp:
                 modes += list(m)
-    kcs = list(kc) if kc is not None else [-1] * len(steps)
-    tls = list(tiles) if tiles is not None else [-1] * len(steps)
+        if kind == 0:
+            kcs.append(kcs_in[ci] if kcs_in else -1)
+            tls.append(tls_in[ci] if tls_in else -1)
+            ci += 1
+        else:
+            kcs.append(-1)
+            tls.append(-1)
     keep = dict(
         slot_elems=_arr(ct.c_int64, slot_elems), slot_batched=_arr(ct.c_int32, slot_batched),
         slot_leaf=_arr(ct.c_int64, slot_leaf), lss=_arr(ct.c_int64, lss or [0]),
         step_slots=_arr(ct.c_int32, step_slots), kc=_arr(ct.c_int32, kcs), tile=_arr(ct.c_int32, tls),
-        nmodes=_arr(ct.c_int32, nmodes), modes=_arr(ct.c_int64, modes))
+        nmodes=_arr(ct.c_int32, nmodes), modes=_arr(ct.c_int64, modes or [0]),
+        dt=_arr(ct.c_int32, dts), kind=_arr(ct.c_int32, kinds))
     d = PlanDesc(len(slots), keep["slot_elems"], keep["slot_batched"], keep["slot_leaf"], n_sliced, keep["lss"],
                  group_bits, len(steps), keep["step_slots"], keep["kc"], keep["tile"], keep["nmodes"],
-                 keep["modes"], out_slot)
+                 keep["modes"], out_slot, keep["dt"], keep["kind"])
     p = ct.c_void_p()
     ctx.check(ctx.lib.pq_plan_create(ctx.ptr, ct.cast(ct.pointer(d), ct.c_void_p), ct.byref(p)))
-    return Plan(ctx, p, slots, steps, out_slot, n_sliced, group_bits)
+    return Plan(ctx, p, slots, steps, out_slot, n_sliced, group_bits, dts)

@@ -288,7 +288,7 @@ def _plan_masks(network, plan):
         c = ma ^ mb
         counter += 1
         tens[f"node_{counter}"] = (c, sa | sb)
-        steps.append((c, k, sa | sb))
+        steps.append((c, k, sa | sb, ma, mb))
     return steps, ids, ext
 
 
@@ -301,21 +301,34 @@ def _mask_size(mask, ext):
     return s
 
 
-def sliced_cost(steps, ext, smask, nslices):
-    """Total MACs over all slices (independent steps once) and the largest
-    per-slice intermediate."""
+# Roofline time model used to rank slicings (seconds): complex64 on MI355X
+# with effective rates below the datasheet peaks, plus a per-launch cost.
+ROOFLINE = {"flops": 100e12, "bytes": 4e12, "launch": 4e-6, "esize": 8}
+
+
+def sliced_cost(steps, ext, smask, nslices, model="macs"):
+    """Total cost over all slices (steps that do not depend on a sliced bond
+    run once) and the largest per-slice intermediate.  model="macs": complex
+    multiply-adds; model="time": roofline seconds (ROOFLINE)."""
     total = 0
     peak = 0
-    for c, k, sub in steps:
+    R = ROOFLINE
+    for c, k, sub, ma, mb in steps:
         size_c = _mask_size(c & ~smask, ext)
         size_k = _mask_size(k & ~smask, ext)
         mult = nslices if (sub & smask) else 1
-        total += size_c * size_k * mult
+        if model == "macs":
+            total += size_c * size_k * mult
+        else:
+            size_a = _mask_size(ma & ~smask, ext)
+            size_b = _mask_size(mb & ~smask, ext)
+            t = max(8.0 * size_c * size_k / R["flops"], R["esize"] * (size_a + size_b + size_c) / R["bytes"])
+            total += t * mult + R["launch"]
         peak = max(peak, size_c)
     return total, peak
 
 
-def choose_slices(network, plan, n_bonds, max_elems=2 ** 28, exclude=(), candidates=200):
+def choose_slices(network, plan, n_bonds, max_elems=2 ** 28, exclude=(), candidates=200, model="time"):
     """Greedy choice of `n_bonds` extent-2 bonds to slice: each round adds the
     bond that minimises the total sliced MACs (steps that do not depend on a
     sliced bond run once, the others once per slice), preferring choices that
@@ -332,7 +345,7 @@ def choose_slices(network, plan, n_bonds, max_elems=2 ** 28, exclude=(), candida
     cand = []
     seen = set()
     for i in order:
-        c, k, _ = steps[i]
+        c, k = steps[i][0], steps[i][1]
         m = c | k
         while m:
             low = m & -m
@@ -353,7 +366,7 @@ def choose_slices(network, plan, n_bonds, max_elems=2 ** 28, exclude=(), candida
         for j in cand:
             if smask >> j & 1:
                 continue
-            tot, peak = sliced_cost(steps, ext, smask | (1 << j), ns)
+            tot, peak = sliced_cost(steps, ext, smask | (1 << j), ns, model)
             key = (peak > max_elems, tot, peak)
             if best is None or key < best[0]:
                 best = (key, j)
@@ -361,5 +374,5 @@ def choose_slices(network, plan, n_bonds, max_elems=2 ** 28, exclude=(), candida
             raise ValueError("not enough bonds to slice")
         chosen.append(best[1])
         smask |= 1 << best[1]
-    tot, peak = sliced_cost(steps, ext, smask, 2 ** n_bonds)
+    tot, peak = sliced_cost(steps, ext, smask, 2 ** n_bonds, "macs")
     return [inv[j] for j in chosen], tot, peak

@@ -32,7 +32,7 @@ def gemm(ctx, A, ia, B, ib, tile=-1):
     return out
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("tile", list(range(12)))
 def test_tiles_and_access_orders(ctx, tile):
     """C[m,n] = A[m,k] B[k,n] with K fastest / slowest in each operand."""
     rng = np.random.default_rng(tile)

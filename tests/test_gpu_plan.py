@@ -1,7 +1,12 @@
 """Planned executor (pq_plan_*): whole plans over all slices with the FP64
 slice sum, against brute-force state vectors, the committed config-2
 amplitudes and the reference driver path on the same backend.
-Tolerances: ComplexF64 1e-12 relative; ComplexF32 1e-5 relative."""
+
+Tolerances (worst relative amplitude error):
+  ComplexF64 ...................................... 1e-12
+  complex64 everywhere (gate tensors rounded too) .. 5e-5
+  mixed: FP64 leaves and small steps, complex64 on FP32 MFMA for the
+  large steps, FP64 slice sum (the bench path) ..... 1e-5"""
 import json
 
 import numpy as np
@@ -15,7 +20,7 @@ from picoquant_hip.executor import compile_plan
 
 pytestmark = pytest.mark.gpu
 
-RTOL = {np.complex128: 1e-12, np.complex64: 1e-5}
+RTOL = {np.complex128: 1e-12, np.complex64: 5e-5, "mixed": 1e-5}
 
 
 @pytest.fixture(scope="module", params=[np.complex128, np.complex64], ids=["c128", "c64"])
@@ -129,3 +134,24 @@ def test_plan_rejects_bad_descriptors():
         compile_plan(tn, plan[:-1])                    # leaves two tensors
     with pytest.raises(_lib.PQError):
         compile_plan(tn, plan, group_bits=0, kc=[_lib.KC_GEMM] * len(plan))   # M, N < 16
+
+
+@pytest.mark.parametrize("nslice,group_bits", [(0, 0), (6, 3), (6, 6)])
+def test_grid5x5_d12_mixed_precision(golden_dir, nslice, group_bits):
+    """Config 2 on the bench's precision split: FP64 world lines, complex64
+    sweep on FP32 MFMA, FP64 slice sum -- within 1e-5 of the FP64 amplitudes."""
+    g = json.load(open(golden_dir / "grid5x5_d12.json"))
+    cfg = list(g["batch_base"])
+    for q in g["open_qubits_1based"]:
+        cfg[q - 1] = "x"
+    tn = grid_net(5, 5, 12, g["seed"], "".join(cfg), np.complex128)
+    plan = PL.sweep_plan(tn, PL.grid_site_order(5, 5, open_last=g["open_qubits_1based"]))
+    sliced = PL.choose_slices(tn, plan, nslice)[0] if nslice else []
+    p = compile_plan(tn, plan, sliced, group_bits=group_bits, precision="mixed")
+    assert {d for d in p.dtypes} == {0, 1}                 # both element types in use
+    assert any(k == 1 for k, *_ in p.steps)                # with conversions
+    want = np.array([complex(a, b) for a, b in g["batch_amplitudes"]])
+    for _ in range(2):
+        p.execute(True)
+        assert rel(p.result(), want) < RTOL["mixed"]
+    p.close()
