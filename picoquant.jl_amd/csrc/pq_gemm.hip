@@ -172,6 +172,191 @@ __global__ __launch_bounds__(256) void k_gemm(const C* __restrict__ A, const C* 
       }
 }
 
+// ------------------------------------------------------------------ k_gemm2
+// Fast path when the summed indices run contiguously in both operands
+// (first K mode has unit stride in A and in B and its extent is a multiple
+// of BKT) and every offset keeps 16-byte alignment: each operand row (A) or
+// column (B) of a K tile is one contiguous run, read with 16-byte vector
+// loads.  The next tile is prefetched into registers while the matrix
+// cores work on the current one (register double buffering).
+template <typename C, int BM, int BN, int WGM, int BKT>
+__global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C* __restrict__ B,
+                                               C* __restrict__ Out, GemmArgs g) {
+  typedef typename real_of<C>::type R;
+  typedef mfma_t<R> MF;
+  constexpr int WGN = 4 / WGM;
+  constexpr int WM = BM / WGM, WN = BN / WGN;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  static_assert(WM % 16 == 0 && WN % 16 == 0, "wave tile must be a multiple of 16");
+  constexpr int VEC = 16 / (int)sizeof(C);       // complex elements per 16-byte chunk
+  constexpr int CPR = BKT / VEC;                  // chunks per row of a K tile
+  constexpr int ACH = (BM * CPR + 255) / 256;     // chunks per lane, A tile
+  constexpr int BCH = (BN * CPR + 255) / 256;     // chunks per lane, B tile
+  constexpr int KT_MAX = 2048 / BKT;              // K tiles (K <= 2048)
+
+  __shared__ R As_r[BKT][BM + 4], As_i[BKT][BM + 4];
+  __shared__ R Bs_r[BKT][BN + 4], Bs_i[BKT][BN + 4];
+  __shared__ int32_t offAm[BM], offCm[BM], offBn[BN], offCn[BN];
+  __shared__ int32_t offAk[KT_MAX], offBk[KT_MAX];
+
+  const uint32_t Mt = g.M.total, Nt = g.N.total, Kt = g.K.total;
+  const uint32_t m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int tid = threadIdx.x;
+  {
+    int64_t oa, ob, oc;
+    batch_offsets(g.L, blockIdx.z, oa, ob, oc);
+    A += oa;
+    B += ob;
+    Out += oc;
+  }
+  for (int t = tid; t < BM; t += 256) {
+    int32_t oa = 0, oc = 0;
+    if (m0 + t < Mt) group_offsets(g.M, m0 + t, oa, oc);
+    offAm[t] = oa;
+    offCm[t] = oc;
+  }
+  for (int t = tid; t < BN; t += 256) {
+    int32_t ob = 0, oc = 0;
+    if (n0 + t < Nt) group_offsets(g.N, n0 + t, ob, oc);
+    offBn[t] = ob;
+    offCn[t] = oc;
+  }
+  const uint32_t ntiles = Kt / BKT;
+  for (uint32_t t = tid; t < ntiles; t += 256) {
+    int32_t oa, ob;
+    group_offsets(g.K, t * BKT, oa, ob);
+    offAk[t] = oa;
+    offBk[t] = ob;
+  }
+  __syncthreads();
+
+  const int wave = tid / 64, lane = tid % 64;
+  const int wm = (wave / WGN) * WM, wn = (wave % WGN) * WN;
+  const int li = lane % 16, lk = lane / 16;
+
+  typename MF::acc accr[TM][TN], acci[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        accr[i][j][r] = 0;
+        acci[i][j][r] = 0;
+      }
+
+  float4 ra[ACH], rb[BCH];
+  auto load_tile = [&](uint32_t kt) {
+#pragma unroll
+    for (int q = 0; q < ACH; ++q) {
+      const int id = tid + 256 * q;
+      const int row = id / CPR, cc = id % CPR;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (row < BM && m0 + row < Mt)
+        v = *reinterpret_cast<const float4*>(A + offAm[row] + offAk[kt] + cc * VEC);
+      ra[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < BCH; ++q) {
+      const int id = tid + 256 * q;
+      const int col = id / CPR, cc = id % CPR;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (col < BN && n0 + col < Nt)
+        v = *reinterpret_cast<const float4*>(B + offBn[col] + offBk[kt] + cc * VEC);
+      rb[q] = v;
+    }
+  };
+  auto store_tile = [&]() {
+#pragma unroll
+    for (int q = 0; q < ACH; ++q) {
+      const int id = tid + 256 * q;
+      const int row = id / CPR, k = (id % CPR) * VEC;
+      if (row < BM) {
+        if constexpr (VEC == 2) {
+          As_r[k][row] = ra[q].x;
+          As_i[k][row] = ra[q].y;
+          As_r[k + 1][row] = ra[q].z;
+          As_i[k + 1][row] = ra[q].w;
+        } else {
+          const double2 d = *reinterpret_cast<const double2*>(&ra[q]);
+          As_r[k][row] = d.x;
+          As_i[k][row] = d.y;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < BCH; ++q) {
+      const int id = tid + 256 * q;
+      const int col = id / CPR, k = (id % CPR) * VEC;
+      if (col < BN) {
+        if constexpr (VEC == 2) {
+          Bs_r[k][col] = rb[q].x;
+          Bs_i[k][col] = rb[q].y;
+          Bs_r[k + 1][col] = rb[q].z;
+          Bs_i[k + 1][col] = rb[q].w;
+        } else {
+          const double2 d = *reinterpret_cast<const double2*>(&rb[q]);
+          Bs_r[k][col] = d.x;
+          Bs_i[k][col] = d.y;
+        }
+      }
+    }
+  };
+
+  load_tile(0);
+  for (uint32_t kt = 0; kt < ntiles; ++kt) {
+    store_tile();
+    __syncthreads();
+    if (kt + 1 < ntiles) load_tile(kt + 1);
+#pragma unroll
+    for (int kk = 0; kk < BKT; kk += 4) {
+      R ar[TM], ai[TM], br[TN], bi[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        ar[i] = As_r[kk + lk][wm + 16 * i + li];
+        ai[i] = As_i[kk + lk][wm + 16 * i + li];
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        br[j] = Bs_r[kk + lk][wn + 16 * j + li];
+        bi[j] = Bs_i[kk + lk][wn + 16 * j + li];
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          accr[i][j] = MF::op(ar[i], br[j], accr[i][j]);
+          accr[i][j] = MF::op(-ai[i], bi[j], accr[i][j]);
+          acci[i][j] = MF::op(ar[i], bi[j], acci[i][j]);
+          acci[i][j] = MF::op(ai[i], br[j], acci[i][j]);
+        }
+    }
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int m = wm + 16 * i + MF::row(lk, r);
+        int n = wn + 16 * j + li;
+        if (m0 + m < Mt && n0 + n < Nt) {
+          C v;
+          v.x = accr[i][j][r];
+          v.y = acci[i][j][r];
+          Out[offCm[m] + offCn[n]] = v;
+        }
+      }
+}
+
+// K-tile depth of the fast path: 32 for complex64 tiles whose LDS allows it.
+template <typename C, int BM, int BN>
+constexpr int fast_bk() {
+  return (sizeof(C) == 8 && BM + BN <= 192) ? 32 : 16;
+}
+
 // Static LDS stays within 64 KiB: complex128 tiles up to 128 rows or
 // columns and 4096 elements.
 template <typename C, int BM, int BN>
@@ -185,6 +370,24 @@ hipError_t launch_tile(const C* A, const C* B, C* Out, const GemmArgs& g, hipStr
     return launch_tile<C, (BM > 64 ? 64 : BM), (BN > 64 ? 64 : BN), WGM>(A, B, Out, g, s);
   } else {
     dim3 grid((g.M.total + BM - 1) / BM, (g.N.total + BN - 1) / BN, g.L.total);
+    constexpr int BKF = fast_bk<C, BM, BN>();
+    // K tile depth: the deepest of 32 / 16 / 8 that divides the first
+    // (contiguous) K mode, so a tile never straddles two runs.
+    const uint32_t k0ext = g.K.nmodes > 0 ? g.K.ext[0].d : 0;
+    if (g.contig && k0ext > 0) {
+      if (k0ext % BKF == 0) {
+        hipLaunchKernelGGL((k_gemm2<C, BM, BN, WGM, BKF>), grid, dim3(256), 0, s, A, B, Out, g);
+        return hipGetLastError();
+      }
+      if (BKF > 16 && k0ext % 16 == 0) {
+        hipLaunchKernelGGL((k_gemm2<C, BM, BN, WGM, 16>), grid, dim3(256), 0, s, A, B, Out, g);
+        return hipGetLastError();
+      }
+      if (k0ext % 8 == 0) {
+        hipLaunchKernelGGL((k_gemm2<C, BM, BN, WGM, 8>), grid, dim3(256), 0, s, A, B, Out, g);
+        return hipGetLastError();
+      }
+    }
     if (g.a_kfast && g.b_kfast)
       hipLaunchKernelGGL((k_gemm<C, BM, BN, WGM, true, true>), grid, dim3(256), 0, s, A, B, Out, g);
     else if (g.a_kfast)

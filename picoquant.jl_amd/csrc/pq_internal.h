@@ -37,7 +37,8 @@ struct GemmArgs {
   Batch L;            // batch entries on grid z
   int32_t a_kfast;    // A's unit-stride mode is in K: lanes walk k when loading
   int32_t b_kfast;    // B's unit-stride mode is in K
-  int32_t tile;       // 0:32x32 1:64x32 2:32x64 3:64x64 4:128x64 5:64x128
+  int32_t tile;       // see pq_set_gemm_tile
+  int32_t contig;     // first K mode unit-stride in A and B, 16-byte aligned runs
 };
 
 // One problem of a grouped launch of small contractions (PQ_KC_BATCHED):

@@ -300,6 +300,18 @@ static int launch_typed(pq_ctx* c, const C* pa, const C* pb, C* pc, const StepSp
       if (m.s1 == 1) ga.b_kfast = 1;
     }
     ga.tile = s.tile;
+    // Vector-load fast path: K runs contiguously (unit stride) in both
+    // operands and every element offset keeps 16-byte alignment.
+    {
+      const int64_t vec = 16 / (int64_t)sizeof(C);
+      bool ok = !s.K.empty() && s.K[0].s0 == 1 && s.K[0].s1 == 1 && s.K[0].ext % 8 == 0 &&
+                ((uintptr_t)pa % 16) == 0 && ((uintptr_t)pb % 16) == 0;
+      for (size_t q = 1; q < s.K.size() && ok; ++q) ok = s.K[q].s0 % vec == 0 && s.K[q].s1 % vec == 0;
+      for (auto& m : s.M) ok = ok && m.s0 % vec == 0;
+      for (auto& m : s.N) ok = ok && m.s0 % vec == 0;
+      for (int j = 0; j < s.L.nmodes && ok; ++j) ok = s.L.sA[j] % vec == 0 && s.L.sB[j] % vec == 0;
+      ga.contig = ok ? 1 : 0;
+    }
     ProfScope ps(c, PQ_KC_GEMM, alg_bytes, alg_flops);
     e = launch_gemm<C>(pa, pb, pc, ga, c->stream);
   } else {

@@ -122,12 +122,17 @@ def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf
                 raise ValueError("output order does not match the open indices")
             C.idx = list(order)
         elif sc in consumer:
-            partner = set(slots[consumer[sc]].idx) if consumer[sc] < sc else None
-            if partner is None:
-                # partner produced later: its index set is known from pass 1
-                partner = set(slots[consumer[sc]].idx)
-            first = [x for x in C.idx if x in partner]
-            C.idx = first + [x for x in C.idx if x not in partner]
+            pslot = slots[consumer[sc]]
+            cset = set(C.idx)
+            if consumer[sc] < sc:
+                # partner already laid out: take the summed indices in its
+                # order, so K is contiguous in the same order in both operands
+                first = [x for x in pslot.idx if x in cset]
+            else:
+                pset = set(pslot.idx)
+                first = [x for x in C.idx if x in pset]
+            fset = set(first)
+            C.idx = first + [x for x in C.idx if x not in fset]
         st = 1
         for x in C.idx:
             C.stride[x] = st
