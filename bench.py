@@ -58,10 +58,49 @@ def build_network(dtype, backend=None):
     return tn, open_q
 
 
-def choose_plan(tn, open_q):
-    from picoquant_hip import planner as PL
+def train_cost_model(ctx, n_samples=20000):
+    """The learned cost model, trained here on this GPU: ~20k contraction
+    steps drawn from candidate plans of 20-36-qubit grid circuits (seeds
+    11-14, not the benchmark circuit), timed with pq_time_step, fitted with
+    the HIP MLP kernels.  Returns (model, stats)."""
+    from picoquant_hip import costmodel as CM
+    t0 = time.perf_counter()
+    plans = CM.grid_training_plans()
+    X, us, keys = CM.collect_dataset(ctx, plans, n_samples=n_samples, seed=0)
+    t1 = time.perf_counter()
+    uniq = np.unique(keys)
+    rng = np.random.default_rng(0)
+    held = set(rng.choice(uniq, size=len(uniq) // 5, replace=False).tolist())
+    te = np.array([k in held for k in keys])
+    model = CM.MLPCostModel(ctx, hidden=64, seed=0)
+    model.fit(X[~te], us[~te], epochs=150)
+    sel = te & (us > 12.0)
+    pred = model.predict_log_us(X[sel])
+    rp = np.argsort(np.argsort(pred))
+    rt = np.argsort(np.argsort(np.log(us[sel])))
+    stats = {"samples": int(len(X)), "unique_shapes": int(len(uniq)),
+             "holdout_spearman": float(np.corrcoef(rp, rt)[0, 1]),
+             "holdout_median_abs_ratio_err": float(np.median(np.abs(np.exp(pred - np.log(us[sel])) - 1))),
+             "collect_s": round(t1 - t0, 1)}
+    final = CM.MLPCostModel(ctx, hidden=64, seed=0)       # deployed model: all samples
+    stats["train_loss"] = float(final.fit(X, us, epochs=150))
+    stats["fit_s"] = round(time.perf_counter() - t1, 1)
+    return final, stats
+
+
+def choose_plan(tn, open_q, model=None):
     from picoquant_hip import learned
-    return learned.select_plan(tn, ROWS, COLS, open_q, N_SLICED)
+    return learned.select_plan(tn, ROWS, COLS, open_q, N_SLICED, model=model)
+
+
+def time_plan(p, ctx, runs):
+    p.execute(True)
+    ctx.synchronize()
+    t = time.perf_counter()
+    for _ in range(runs):
+        p.execute(True)
+    ctx.synchronize()
+    return (time.perf_counter() - t) / runs
 
 
 def plain_fp64_slice_time(plan, sliced, open_q):
@@ -138,17 +177,21 @@ def main():
     from picoquant_hip import _lib
     from picoquant_hip.executor import compile_plan
 
+    from picoquant_hip import autotune
     t0 = time.perf_counter()
     tn, open_q = build_network(np.complex128)
-    sel = choose_plan(tn, open_q)
+    ctx = tn.backend.ctx
+    model, mstats = train_cost_model(ctx)
+    log(f"[bench] cost model {time.perf_counter() - t0:.1f}s: {mstats}")
+    sel = choose_plan(tn, open_q, model)
     plan, sliced = sel["plan"], sel["sliced"]
-    log(f"[bench] network+plan {time.perf_counter() - t0:.1f}s: {sel['summary']}")
-    p = compile_plan(tn, plan, sliced, group_bits=sel.get("group_bits"), kc=sel.get("kc"),
-                     tiles=sel.get("tiles"), precision="mixed")
+    log(f"[bench] plan {time.perf_counter() - t0:.1f}s: {sel['summary']}")
+    kcs, tiles, tuned = autotune.autotune_plan(ctx, tn, plan, sliced, group_bits=N_SLICED)
+    log(f"[bench] autotuned {len(tuned)} steps {time.perf_counter() - t0:.1f}s")
+    p = compile_plan(tn, plan, sliced, group_bits=N_SLICED, kc=kcs, tiles=tiles, precision="mixed")
     st = p.stats()
     log(f"[bench] compiled: arena {st['arena_bytes'] / 2**30:.2f} GiB, {st['launches']} launches/run, "
         f"group_bits {p.group_bits}")
-    ctx = tn.backend.ctx
     for _ in range(args.warmup):
         p.execute(True)
     ctx.synchronize()
@@ -187,9 +230,23 @@ def main():
     roof["classes"] = {_lib.KC_NAMES[k]: round(v["ms"], 3) for k, v in per.items()}
     p.close()
 
+    # What the learned model buys: the pick FLOP counting alone would make,
+    # compiled the same way and timed.
+    planner_info = {"cost_model": mstats, "selection": sel["summary"], "learned_ms": 1e3 * dt / args.steps}
+    if sel["flop_plan"] is not sel["plan"]:
+        kf, tf, _ = autotune.autotune_plan(ctx, tn, sel["flop_plan"], sel["flop_sliced"], group_bits=N_SLICED)
+        pf = compile_plan(tn, sel["flop_plan"], sel["flop_sliced"], group_bits=N_SLICED, kc=kf, tiles=tf,
+                          precision="mixed")
+        planner_info["flop_pick_ms"] = 1e3 * time_plan(pf, ctx, 3)
+        pf.close()
+    else:
+        planner_info["flop_pick_ms"] = planner_info["learned_ms"]
+    planner_info["learned_gain"] = planner_info["flop_pick_ms"] / planner_info["learned_ms"]
+    log(f"[bench] planner: {planner_info}")
+
     # FP64 run of the same plan: the accuracy reference for the complex64 path.
     tn64 = tn
-    p64 = compile_plan(tn64, plan, sliced, group_bits=sel.get("group_bits"))
+    p64 = compile_plan(tn64, plan, sliced, group_bits=N_SLICED)
     p64.execute(True)
     ref = p64.result().reshape(-1)
     ctx64 = tn64.backend.ctx
@@ -238,7 +295,7 @@ def main():
         "plain_fp64_gpu": plain,
         "speedup_vs_plain": (amps / plain["amps_per_s"]) if plain else None,
         "batched_fp64_amps_per_s": 64 / t_fp64_batched,
-        "plan": sel["summary"],
+        "planner": planner_info,
     }
     print(json.dumps(out))
 

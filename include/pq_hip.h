@@ -167,6 +167,34 @@ int pq_plan_step_info(const pq_plan* plan, int step, int* kclass, int* tile, int
 int pq_plan_profile(pq_ctx* ctx, pq_plan* plan, double* step_ms);
 int pq_plan_destroy(pq_ctx* ctx, pq_plan* plan);
 
+/* ------------------------------------------------------------------------
+ * Cost model and autotuning (no counterpart in the reference, whose planners
+ * rank plans by FLOP counts only, src/layer2/bgreedy_contraction.jl:53-66).
+ * ---------------------------------------------------------------------- */
+/* Time one pairwise contraction shape: modes as in pq_plan_desc (M, N, K
+ * triples), `batch` entries packed one after another; kclass / tile as for
+ * pq_set_kernel_override / pq_set_gemm_tile (-1 = automatic).  Writes the
+ * median of `reps` event-timed launches in microseconds. */
+int pq_time_step(pq_ctx* ctx, int dtype, int nm, int nn, int nk, const int64_t* modes, int batch,
+                 int kclass, int tile, int reps, double* us);
+
+/* 3-layer perceptron (in -> hidden -> hidden -> 1, ReLU, FP32) with HIP
+ * forward, backward and Adam kernels.  Parameters as one FP32 vector:
+ * W1[in][H] b1[H] W2[H][H] b2[H] w3[H] b3.  X is row-major [n][in]. */
+typedef struct pq_mlp pq_mlp;
+int pq_mlp_create(pq_ctx* ctx, int in_dim, int hidden, uint64_t seed, pq_mlp** out);
+int pq_mlp_destroy(pq_mlp* mlp);
+int pq_mlp_num_params(const pq_mlp* mlp);
+int pq_mlp_get_params(pq_ctx* ctx, pq_mlp* mlp, float* host);
+int pq_mlp_set_params(pq_ctx* ctx, pq_mlp* mlp, const float* host);
+int pq_mlp_predict(pq_ctx* ctx, pq_mlp* mlp, const float* X, int n, float* out);
+/* Mean squared error over the n samples and its gradient (np floats). */
+int pq_mlp_loss_grad(pq_ctx* ctx, pq_mlp* mlp, const float* X, const float* y, int n, float* loss,
+                     float* grad);
+/* Adam on shuffled minibatches; final_loss = mean loss of the last epoch. */
+int pq_mlp_train(pq_ctx* ctx, pq_mlp* mlp, const float* X, const float* y, int n, int epochs, int batch,
+                 float lr, uint64_t seed, float* final_loss);
+
 #ifdef __cplusplus
 }
 #endif

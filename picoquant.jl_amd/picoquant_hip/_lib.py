@@ -61,6 +61,18 @@ SIGNATURES = {
                                      ct.POINTER(ct.c_int), ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
     "pq_plan_profile": (ct.c_int, [_P, _P, ct.POINTER(ct.c_double)]),
     "pq_plan_destroy": (ct.c_int, [_P, _P]),
+    "pq_time_step": (ct.c_int, [_P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _I64P, ct.c_int, ct.c_int, ct.c_int,
+                                ct.c_int, ct.POINTER(ct.c_double)]),
+    "pq_mlp_create": (ct.c_int, [_P, ct.c_int, ct.c_int, ct.c_uint64, ct.POINTER(_P)]),
+    "pq_mlp_destroy": (ct.c_int, [_P]),
+    "pq_mlp_num_params": (ct.c_int, [_P]),
+    "pq_mlp_get_params": (ct.c_int, [_P, _P, ct.POINTER(ct.c_float)]),
+    "pq_mlp_set_params": (ct.c_int, [_P, _P, ct.POINTER(ct.c_float)]),
+    "pq_mlp_predict": (ct.c_int, [_P, _P, ct.POINTER(ct.c_float), ct.c_int, ct.POINTER(ct.c_float)]),
+    "pq_mlp_loss_grad": (ct.c_int, [_P, _P, ct.POINTER(ct.c_float), ct.POINTER(ct.c_float), ct.c_int,
+                                    ct.POINTER(ct.c_float), ct.POINTER(ct.c_float)]),
+    "pq_mlp_train": (ct.c_int, [_P, _P, ct.POINTER(ct.c_float), ct.POINTER(ct.c_float), ct.c_int, ct.c_int,
+                                ct.c_int, ct.c_float, ct.c_uint64, ct.POINTER(ct.c_float)]),
 }
 
 _lib = None
@@ -198,6 +210,15 @@ class Context:
 
     def set_gemm_tile(self, tile):
         self.check(self.lib.pq_set_gemm_tile(self.ptr, int(tile)))
+
+    def time_step(self, M, N, K, batch=1, kclass=-1, tile=-1, reps=3, dtype=None):
+        """Median microseconds of one contraction shape (modes as (ext, s0, s1))."""
+        code = self.code if dtype is None else dtype
+        modes = [x for grp in (M, N, K) for m in grp for x in m]
+        us = ct.c_double()
+        self.check(self.lib.pq_time_step(self.ptr, code, len(M), len(N), len(K), i64arr(modes or [0]), batch,
+                                         kclass, tile, reps, ct.byref(us)))
+        return us.value
 
     def profile_enable(self, on=True):
         self.check(self.lib.pq_profile_enable(self.ptr, 1 if on else 0))

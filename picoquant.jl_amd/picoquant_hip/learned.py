@@ -1,12 +1,12 @@
-"""Plan selection: candidate contraction orders + slicings, ranked by a cost
-model; the reference's greedy planner (bgreedy, tau -> 0) is always a
-candidate and the fallback.
+"""Plan selection with the learned cost model.
 
-Candidates: world-line + sweep plans over several site orders (column/row
-snakes and seeded perturbations of them) and Boltzmann-greedy samples.  Each
-candidate gets the greedy slicing of planner.choose_slices.  The ranking uses
-the trained MLP cost model when one is loaded (predicted kernel time per
-step, summed), otherwise the exact sliced MAC count.
+For a grid-circuit network: generate candidate contraction orders (world-line
++ sweep plans over column / row snakes and seeded perturbations of them, plus
+Boltzmann-greedy samples of the reference's planner), rank all of them by the
+MLP's predicted run time, slice a shortlist (planner.choose_slices) and rank
+the sliced plans again.  The reference's greedy plan (bgreedy, tau -> 0) is
+always a candidate and is the fallback when no candidate meets the slice
+size cap.  The pick the FLOP count alone would make is kept for comparison.
 """
 from __future__ import annotations
 
@@ -17,55 +17,87 @@ import numpy as np
 from . import planner as PL
 
 
-def _perturbed_order(order, rng, swaps):
+def _perturbed(order, rng, swaps, keep_last):
     o = list(order)
     for _ in range(swaps):
         i = int(rng.integers(0, len(o) - 1))
         o[i], o[i + 1] = o[i + 1], o[i]
-    return o
+    ks = set(keep_last)
+    return [q for q in o if q not in ks] + [q for q in o if q in ks]
 
 
-def candidate_plans(tn, rows, cols, open_q, n_candidates=8, seed=0, with_greedy=True):
-    """[(name, plan)] candidate pair plans for a grid network."""
+def candidate_plans(tn, rows, cols, open_q, n_candidates=64, seed=0, n_greedy=2):
+    """[(name, plan)]: n_candidates candidate pair plans for a grid network."""
     rng = np.random.default_rng(seed)
-    cands = []
     base = {"col": PL.grid_site_order(rows, cols, "col", open_last=open_q),
             "row": PL.grid_site_order(rows, cols, "row", open_last=open_q)}
-    for name, order in base.items():
-        cands.append((f"sweep-{name}", PL.sweep_plan(tn, order)))
-    if with_greedy:
-        plan, _, _ = PL.bgreedy_single(tn, 1.0, 0.0)
-        cands.append(("bgreedy-tau0", plan))
+    cands = [(f"sweep-{k}", PL.sweep_plan(tn, v)) for k, v in base.items()]
+    for k, v in base.items():
+        rev = list(reversed([q for q in v if q not in set(open_q)])) + [q for q in v if q in set(open_q)]
+        cands.append((f"sweep-{k}-rev", PL.sweep_plan(tn, rev)))
+    plan0, _, _ = PL.bgreedy_single(tn, 1.0, 0.0)
+    cands.append(("bgreedy-tau0", plan0))
+    for g in range(n_greedy):
+        alpha = [0.5, 1.5, 1.0, 2.0][g % 4]
+        plan, _, _ = PL.bgreedy_single(tn, alpha, 1e6 * (g + 1), rng)
+        cands.append((f"bgreedy-a{alpha}-t{g}", plan))
     k = 0
     while len(cands) < n_candidates:
         name = "col" if k % 2 == 0 else "row"
-        order = _perturbed_order(base[name], rng, 1 + k // 2)
-        # keep the open sites last
-        order = [q for q in order if q not in set(open_q)] + [q for q in order if q in set(open_q)]
-        cands.append((f"sweep-{name}-perturb{k}", PL.sweep_plan(tn, order)))
+        swaps = 1 + (k // 2) % 6
+        cands.append((f"sweep-{name}-p{k}", PL.sweep_plan(tn, _perturbed(base[name], rng, swaps, open_q))))
         k += 1
-    return cands
+    return cands[:n_candidates]
 
 
-def select_plan(tn, rows, cols, open_q, n_sliced, n_candidates=6, model=None, max_elems=2 ** 28):
-    """Pick a plan + slicing.  Returns a dict with plan, sliced bonds and a
-    summary of the ranking."""
-    cands = candidate_plans(tn, rows, cols, open_q, n_candidates)
-    scored = []
+def select_plan(tn, rows, cols, open_q, n_sliced, model=None, n_candidates=64, shortlist=8,
+                max_elems=2 ** 28, seed=0):
+    """Pick a plan + slicing.  model: a trained costmodel.MLPCostModel, or
+    None to rank by the roofline-time slicing objective alone."""
+    cands = candidate_plans(tn, rows, cols, open_q, n_candidates, seed)
+    rows_out = []
     for name, plan in cands:
         led = PL.plan_ledger(tn, plan)
-        scored.append((led["macs"], name, plan, led))
-    scored.sort(key=lambda x: x[0])
-    # slice the cheapest few (slicing search is the expensive part)
-    best = None
-    for macs, name, plan, led in scored[:3]:
-        sliced, tot, peak = PL.choose_slices(tn, plan, n_sliced, max_elems=max_elems)
-        score = tot if model is None else model.plan_time(tn, plan, sliced)
-        if best is None or score < best[0]:
-            best = (score, name, plan, sliced, tot, peak, macs)
-    score, name, plan, sliced, tot, peak, macs = best
-    summary = {"choice": name, "ranked_by": "mlp" if model is not None else "sliced MACs",
-               "unsliced_macs_log2": round(math.log2(macs), 3), "sliced_macs_log2": round(math.log2(tot), 3),
-               "peak_slice_elems_log2": round(math.log2(peak), 2),
-               "candidates": [(n, round(math.log2(m), 2)) for m, n, _, _ in scored]}
-    return {"plan": plan, "sliced": sliced, "summary": summary}
+        pred = model.plan_time(tn, plan) if model is not None and led["max_size"] <= 2 ** 34 else math.inf
+        rows_out.append({"name": name, "plan": plan, "macs": led["macs"], "max": led["max_size"], "pred_us": pred})
+    # shortlist: the best predicted (or the cheapest by MACs without a model)
+    key = (lambda r: r["pred_us"]) if model is not None else (lambda r: r["macs"])
+    order = sorted(rows_out, key=key)
+    short = order[:shortlist]
+    by_macs = min(rows_out, key=lambda r: r["macs"])
+    if by_macs not in short:
+        short.append(by_macs)
+    greedy = next(r for r in rows_out if r["name"] == "bgreedy-tau0")
+    for r in short:
+        try:
+            sl, tot, peak = PL.choose_slices(tn, r["plan"], n_sliced, max_elems=max_elems)
+        except ValueError:
+            r["sliced"], r["sliced_macs"], r["peak"], r["pred_sliced_us"] = None, math.inf, math.inf, math.inf
+            continue
+        r["sliced"], r["sliced_macs"], r["peak"] = sl, tot, peak
+        r["pred_sliced_us"] = model.plan_time(tn, r["plan"], sl) if model is not None else tot
+    ok = [r for r in short if r.get("sliced") is not None and r["peak"] <= max_elems]
+    if ok:
+        best = min(ok, key=lambda r: r["pred_sliced_us"])
+        flop_pick = min(ok, key=lambda r: r["sliced_macs"])
+        fallback = False
+    else:
+        sl, tot, peak = PL.choose_slices(tn, greedy["plan"], n_sliced, max_elems=max_elems)
+        greedy.update(sliced=sl, sliced_macs=tot, peak=peak, pred_sliced_us=math.nan)
+        best = flop_pick = greedy
+        fallback = True
+    summary = {
+        "choice": best["name"], "ranked_by": "mlp" if model is not None else "roofline slicing objective",
+        "fallback_greedy": fallback, "candidates": len(rows_out),
+        "unsliced_macs_log2": round(math.log2(best["macs"]), 3),
+        "sliced_macs_log2": round(math.log2(best["sliced_macs"]), 3),
+        "peak_slice_elems_log2": round(math.log2(best["peak"]), 2),
+        "predicted_ms": None if model is None else round(best["pred_sliced_us"] / 1e3, 3),
+        "flop_pick": flop_pick["name"],
+        "greedy_unsliced_macs_log2": round(math.log2(greedy["macs"]), 2),
+        "shortlist": [(r["name"], round(math.log2(r["macs"]), 2),
+                       None if model is None else round(r.get("pred_sliced_us", math.inf) / 1e3, 3))
+                      for r in short],
+    }
+    return {"plan": best["plan"], "sliced": best["sliced"], "summary": summary,
+            "flop_plan": flop_pick["plan"], "flop_sliced": flop_pick["sliced"], "rows": rows_out}

@@ -328,19 +328,85 @@ def sliced_cost(steps, ext, smask, nslices, model="macs"):
     return total, peak
 
 
+def _bond_matrices(steps, ext, cand):
+    """Boolean [steps x candidates] incidence of candidate bonds in C, K, A,
+    B and the subtree of each step, plus log2 sizes of C, K, A, B."""
+    ns, nc = len(steps), len(cand)
+    inC = np.zeros((ns, nc), dtype=bool)
+    inK = np.zeros((ns, nc), dtype=bool)
+    inA = np.zeros((ns, nc), dtype=bool)
+    inB = np.zeros((ns, nc), dtype=bool)
+    inS = np.zeros((ns, nc), dtype=bool)
+    lg = {j: math.log2(ext[j]) for j in ext}
+    lc = np.zeros(ns)
+    lk = np.zeros(ns)
+    la = np.zeros(ns)
+    lb = np.zeros(ns)
+
+    def lsize(mask):
+        t = 0.0
+        while mask:
+            low = mask & -mask
+            t += lg[low.bit_length() - 1]
+            mask ^= low
+        return t
+
+    for i, (c, k, sub, ma, mb) in enumerate(steps):
+        lc[i], lk[i], la[i], lb[i] = lsize(c), lsize(k), lsize(ma), lsize(mb)
+        for q, j in enumerate(cand):
+            bit = 1 << j
+            inC[i, q] = bool(c & bit)
+            inK[i, q] = bool(k & bit)
+            inA[i, q] = bool(ma & bit)
+            inB[i, q] = bool(mb & bit)
+            inS[i, q] = bool(sub & bit)
+    return inC, inK, inA, inB, inS, lc, lk, la, lb
+
+
+def _sliced_costs(mats, chosen, nslices, model):
+    """Cost of adding each candidate to the chosen set (vector over candidates)."""
+    inC, inK, inA, inB, inS, lc, lk, la, lb = mats
+    ch = np.zeros(inC.shape[1], dtype=np.int64)
+    ch[chosen] = 1
+    bc = inC @ ch
+    bk = inK @ ch
+    ba = inA @ ch
+    bb = inB @ ch
+    bs = (inS @ ch) > 0
+    # every candidate bond has extent 2: each sliced bond halves a tensor
+    c = lc[:, None] - bc[:, None] - inC
+    k = lk[:, None] - bk[:, None] - inK
+    dep = bs[:, None] | inS
+    mult = np.where(dep, float(nslices), 1.0)
+    if model == "macs":
+        cost = (np.exp2(c + k) * mult).sum(0)
+    else:
+        R = ROOFLINE
+        a = la[:, None] - ba[:, None] - inA
+        b = lb[:, None] - bb[:, None] - inB
+        t = np.maximum(8.0 * np.exp2(c + k) / R["flops"],
+                       R["esize"] * (np.exp2(a) + np.exp2(b) + np.exp2(c)) / R["bytes"])
+        cost = (t * mult).sum(0) + R["launch"] * len(lc)
+    peak = np.exp2(c.max(0))
+    return cost, peak
+
+
 def choose_slices(network, plan, n_bonds, max_elems=2 ** 28, exclude=(), candidates=200, model="time"):
     """Greedy choice of `n_bonds` extent-2 bonds to slice: each round adds the
-    bond that minimises the total sliced MACs (steps that do not depend on a
+    bond that minimises the total sliced cost (steps that do not depend on a
     sliced bond run once, the others once per slice), preferring choices that
-    meet the per-slice size cap.  Returns (bonds, total_macs, peak)."""
+    meet the per-slice size cap.  model: "time" (roofline seconds) or "macs".
+    Returns (bonds, total_macs, peak)."""
     steps, ids, ext = _plan_masks(network, plan)
+    if n_bonds == 0:
+        tot, peak = sliced_cost(steps, ext, 0, 1, "macs")
+        return [], tot, peak
     inv = {v: k for k, v in ids.items()}
     open_out = set()
     for k, e in network.edges.items():
         if e.src is None or e.dst is None:
             open_out.add(k)
     excl = set(exclude) | open_out
-    # candidate bonds: extent 2, internal, present in the costliest steps
     order = sorted(range(len(steps)), key=lambda i: -_mask_size(steps[i][0], ext) * _mask_size(steps[i][1], ext))
     cand = []
     seen = set()
@@ -358,21 +424,21 @@ def choose_slices(network, plan, n_bonds, max_elems=2 ** 28, exclude=(), candida
                 cand.append(j)
         if len(cand) >= candidates:
             break
+    if len(cand) < n_bonds:
+        raise ValueError("not enough bonds to slice")
+    mats = _bond_matrices(steps, ext, cand)
     chosen = []
-    smask = 0
     for r in range(n_bonds):
-        best = None
-        ns = 2 ** (r + 1)
-        for j in cand:
-            if smask >> j & 1:
-                continue
-            tot, peak = sliced_cost(steps, ext, smask | (1 << j), ns, model)
-            key = (peak > max_elems, tot, peak)
-            if best is None or key < best[0]:
-                best = (key, j)
-        if best is None:
-            raise ValueError("not enough bonds to slice")
-        chosen.append(best[1])
-        smask |= 1 << best[1]
+        cost, peak = _sliced_costs(mats, chosen, 2 ** (r + 1), model)
+        cost = cost.astype(float)
+        cost[chosen] = np.inf
+        over = peak > max_elems
+        key = np.where(over, np.inf, cost)
+        if np.all(np.isinf(key)):
+            key = cost
+        chosen.append(int(np.argmin(key)))
+    smask = 0
+    for q in chosen:
+        smask |= 1 << cand[q]
     tot, peak = sliced_cost(steps, ext, smask, 2 ** n_bonds, "macs")
-    return [inv[j] for j in chosen], tot, peak
+    return [inv[cand[q]] for q in chosen], tot, peak

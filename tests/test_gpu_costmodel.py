@@ -1,0 +1,92 @@
+"""Learned cost model on the GPU: the HIP MLP kernels against the numpy oracle
+(gradients within 1e-4 relative, FP32 vs FP64), training, the fit to real
+kernel timings, and the autotuner."""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as MO
+import picoquant_hip as P
+from picoquant_hip import _lib, autotune, costmodel as CM, circuits as C, planner as PL
+from test_host import RecordingBackend
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = _lib.Context(np.complex128)
+    yield c
+    c.close()
+
+
+def test_mlp_gradient_matches_oracle(ctx):
+    rng = np.random.default_rng(1)
+    m = CM.MLPCostModel(ctx, hidden=64, seed=3)
+    p = (rng.standard_normal(m.n_params()) * 0.2).astype(np.float32)
+    m.set_params(p)
+    X = rng.standard_normal((300, CM.N_FEATURES)).astype(np.float32)
+    y = rng.standard_normal(300).astype(np.float32)
+    loss, g = m.loss_grad(X, y)
+    loss_o, g_o = MO.loss_grad(p.astype(np.float64), X, y, CM.N_FEATURES, 64)
+    assert abs(loss - loss_o) <= 1e-4 * abs(loss_o)
+    assert np.max(np.abs(g - g_o)) <= 1e-4 * np.max(np.abs(g_o))
+    out = m.predict_log_us(X * m.sd + m.mu)          # predict standardises again
+    y_o, _ = MO.forward(p.astype(np.float64), X, CM.N_FEATURES, 64)
+    np.testing.assert_allclose(out, y_o, rtol=1e-4, atol=1e-4)
+
+
+def test_mlp_training_reduces_loss(ctx):
+    rng = np.random.default_rng(2)
+    X = rng.uniform(0, 10, (4000, CM.N_FEATURES)).astype(np.float32)
+    us = np.exp(0.3 * X[:, 0] + 0.1 * X[:, 1] * X[:, 2] / 10 + 0.05 * X[:, 3])
+    m = CM.MLPCostModel(ctx, hidden=64, seed=0)
+    loss = m.fit(X, us, epochs=60, batch=256, lr=3e-3)
+    pred = m.predict_log_us(X)
+    assert loss < 0.05
+    assert np.corrcoef(pred, np.log(us))[0, 1] > 0.98
+
+
+def _grid_plans(rows, cols, depth, seed, nsl):
+    n = rows * cols
+    tn = P.network_from_gates(n, C.grid_circuit_gates(rows, cols, depth, seed), backend=RecordingBackend(),
+                              decompose=True)
+    P.add_input(tn, "0" * n)
+    P.add_output(tn, "0" * n)
+    out = []
+    for axis in ("col", "row"):
+        plan = PL.sweep_plan(tn, PL.grid_site_order(rows, cols, axis))
+        out.append((tn, plan, PL.choose_slices(tn, plan, nsl)[0]))
+    return out
+
+
+def test_cost_model_ranks_held_out_timings(ctx):
+    """Train on most unique shapes, rank the held-out ones (those above the
+    launch floor, where timings carry information) by predicted time."""
+    plans = CM.grid_training_plans(specs=((4, 5, 10, 3), (5, 5, 12, 4), (5, 6, 12, 5)), per_circuit=12,
+                                   slicings=(0, 6, 10))
+    X, us, keys = CM.collect_dataset(ctx, plans, n_samples=12000, seed=0)
+    uniq = np.unique(keys)
+    rng = np.random.default_rng(0)
+    test_keys = set(rng.choice(uniq, size=len(uniq) // 5, replace=False).tolist())
+    te = np.array([k in test_keys for k in keys])
+    m = CM.MLPCostModel(ctx, hidden=64, seed=0)
+    m.fit(X[~te], us[~te], epochs=150)
+    sel = te & (us > 12.0)
+    assert sel.sum() > 50
+    pred = m.predict_log_us(X[sel])
+    r_pred = np.argsort(np.argsort(pred))
+    r_true = np.argsort(np.argsort(np.log(us[sel])))
+    rho = np.corrcoef(r_pred, r_true)[0, 1]
+    assert rho > 0.8, rho
+
+
+def test_autotune_never_slower_than_default(ctx):
+    M = [(1024, 64, 1)]
+    N = [(64, 64, 1024)]
+    K = [(64, 1, 1)]
+    kc, tile, us = autotune.tune_step(ctx, M, N, K, 1, _lib.PQ_C64, reps=5)
+    default = ctx.time_step(M, N, K, batch=1, reps=5, dtype=_lib.PQ_C64)
+    assert us <= default * 1.05
+    tn, plan, sliced = _grid_plans(4, 4, 8, 5, 3)[0]
+    kcs, tiles, report = autotune.autotune_plan(ctx, tn, plan, sliced, group_bits=3)
+    assert len(kcs) == len(plan) == len(tiles)
