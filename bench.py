@@ -161,6 +161,36 @@ def cpu_baseline(plan, sliced, cap_s=20.0):
             "elapsed_s": el}
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "gemm_grid6x6d14_traffic.json"
+
+
+def measured_traffic(roof, d):
+    """HBM bytes per launch of the dominant kernel class from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by the copy-kernel
+    calibration), or None when no such measurement is committed."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    t = json.loads(TRAFFIC_FILE.read_text())
+    if t.get("kernel_class") != roof.get("kernel_class"):
+        return None
+    return {"hbm_bytes_per_run": t["hbm_bytes_per_run"], "alg_bytes_per_run": d["bytes"],
+            "ratio": t["hbm_bytes_per_run"] / d["bytes"], "source": TRAFFIC_FILE.name}
+
+
+def calibrate_copy(mib):
+    """Strided-copy kernel over a known number of bytes: reads and writes
+    mib MiB each per launch, 10 launches."""
+    from picoquant_hip import _lib
+    ctx = _lib.Context(np.complex64)
+    n = (mib << 20) // 8
+    h = ctx.upload(np.zeros(n, dtype=np.complex64))
+    for _ in range(10):
+        out = ctx.materialize(h)
+        ctx.free(out)
+    ctx.synchronize()
+    print(json.dumps({"calibrate_copy_bytes_read": n * 8, "bytes_written": n * 8, "launches": 10}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,7 +198,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-model", action="store_true",
+                    help="rank candidates by the roofline slicing objective instead of training the "
+                         "cost model (for profiler runs; the same plan wins on this circuit)")
+    ap.add_argument("--tuned-cache", default="", metavar="JSON",
+                    help="reuse the autotuner's per-step choices from this file (written when missing), "
+                         "so profiler runs trace only the plan's own kernels")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run the strided-copy kernel over a MIB-MiB complex64 tensor 10 times "
+                         "(scale factor for the HBM byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the slices would shard "
               "across GPUs with one FP64 reduction at the end, which this build does not run.")
@@ -181,12 +223,23 @@ def main():
     t0 = time.perf_counter()
     tn, open_q = build_network(np.complex128)
     ctx = tn.backend.ctx
-    model, mstats = train_cost_model(ctx)
+    if args.no_model:
+        model, mstats = None, {"skipped": "--no-model"}
+    else:
+        model, mstats = train_cost_model(ctx)
     log(f"[bench] cost model {time.perf_counter() - t0:.1f}s: {mstats}")
     sel = choose_plan(tn, open_q, model)
     plan, sliced = sel["plan"], sel["sliced"]
     log(f"[bench] plan {time.perf_counter() - t0:.1f}s: {sel['summary']}")
-    kcs, tiles, tuned = autotune.autotune_plan(ctx, tn, plan, sliced, group_bits=N_SLICED)
+    cache = pathlib.Path(args.tuned_cache) if args.tuned_cache else None
+    if cache is not None and cache.exists():
+        kt = json.loads(cache.read_text())
+        kcs, tiles, tuned = kt["kc"], kt["tiles"], kt["report"]
+        log(f"[bench] autotune choices read from {cache}")
+    else:
+        kcs, tiles, tuned = autotune.autotune_plan(ctx, tn, plan, sliced, group_bits=N_SLICED)
+        if cache is not None:
+            cache.write_text(json.dumps({"kc": kcs, "tiles": tiles, "report": tuned}))
     log(f"[bench] autotuned {len(tuned)} steps {time.perf_counter() - t0:.1f}s")
     p = compile_plan(tn, plan, sliced, group_bits=N_SLICED, kc=kcs, tiles=tiles, precision="mixed")
     st = p.stats()
@@ -224,7 +277,7 @@ def main():
         achieved = d["bytes"] / (d["ms"] * 1e-3)
         roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK["hbm"] / 1e9,
                 "unit": "GB/s", "frac": achieved / PEAK["hbm"]}
-    roof["traffic"] = None
+    roof["traffic"] = measured_traffic(roof, d)
     roof["kernel_class"] = _lib.KC_NAMES[dom]
     roof["class_ms_per_run"] = d["ms"]
     roof["classes"] = {_lib.KC_NAMES[k]: round(v["ms"], 3) for k, v in per.items()}
