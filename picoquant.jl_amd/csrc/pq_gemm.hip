@@ -45,6 +45,38 @@ template <> struct mfma_t<double> {
 constexpr int BK = 16;
 constexpr int KTAB_MAX = 2048;   // K tables held in LDS
 
+// FP32 accumulation in two levels: each K tile's products are summed in a
+// fresh accumulator (at most 16 MFMA roundings), which is then added to the
+// running sum.  A single chain over K = 256 rounds 128 times into a growing
+// sum; two levels cut the rounding error of long sums about 2-3x for 2x the
+// accumulator registers.  FP64 accumulates in one chain.
+template <typename MF, int TM, int TN>
+__device__ __forceinline__ void zero_acc(typename MF::acc (&xr)[TM][TN], typename MF::acc (&xi)[TM][TN]) {
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        xr[i][j][r] = 0;
+        xi[i][j][r] = 0;
+      }
+}
+
+template <typename MF, int TM, int TN>
+__device__ __forceinline__ void add_chunk(typename MF::acc (&accr)[TM][TN], typename MF::acc (&acci)[TM][TN],
+                                          const typename MF::acc (&cr)[TM][TN], const typename MF::acc (&ci)[TM][TN]) {
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        accr[i][j][r] += cr[i][j][r];
+        acci[i][j][r] += ci[i][j][r];
+      }
+}
+
 template <typename C, int BM, int BN, int WGM, bool AK, bool BK_FAST>
 __global__ __launch_bounds__(256) void k_gemm(const C* __restrict__ A, const C* __restrict__ B,
                                               C* __restrict__ Out, GemmArgs g) {
@@ -54,6 +86,7 @@ __global__ __launch_bounds__(256) void k_gemm(const C* __restrict__ A, const C* 
   constexpr int WM = BM / WGM, WN = BN / WGN;    // wave tile
   static_assert(WM % 16 == 0 && WN % 16 == 0, "wave tile must be a multiple of 16");
   constexpr int TM = WM / 16, TN = WN / 16;      // 16x16 tiles per wave
+  constexpr bool CHUNKED = sizeof(R) == 4;
 
   __shared__ R As_r[BK][BM + 4], As_i[BK][BM + 4];
   __shared__ R Bs_r[BK][BN + 4], Bs_i[BK][BN + 4];
@@ -96,17 +129,11 @@ __global__ __launch_bounds__(256) void k_gemm(const C* __restrict__ A, const C* 
   const int li = lane % 16, lk = lane / 16;
 
   typename MF::acc accr[TM][TN], acci[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        accr[i][j][r] = 0;
-        acci[i][j][r] = 0;
-      }
+  zero_acc<MF, TM, TN>(accr, acci);
 
   for (uint32_t k0 = 0; k0 < Kt; k0 += BK) {
+    typename MF::acc cr[TM][TN], ci[TM][TN];
+    if constexpr (CHUNKED) zero_acc<MF, TM, TN>(cr, ci);
     // A tile BM x BK -> As[k][m]
     for (int t = tid; t < BM * BK; t += 256) {
       int m, k;
@@ -145,12 +172,15 @@ __global__ __launch_bounds__(256) void k_gemm(const C* __restrict__ A, const C* 
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
-          accr[i][j] = MF::op(ar[i], br[j], accr[i][j]);
-          accr[i][j] = MF::op(-ai[i], bi[j], accr[i][j]);
-          acci[i][j] = MF::op(ar[i], bi[j], acci[i][j]);
-          acci[i][j] = MF::op(ai[i], br[j], acci[i][j]);
+          typename MF::acc& xr = CHUNKED ? cr[i][j] : accr[i][j];
+          typename MF::acc& xi = CHUNKED ? ci[i][j] : acci[i][j];
+          xr = MF::op(ar[i], br[j], xr);
+          xr = MF::op(-ai[i], bi[j], xr);
+          xi = MF::op(ar[i], bi[j], xi);
+          xi = MF::op(ai[i], br[j], xi);
         }
     }
+    if constexpr (CHUNKED) add_chunk<MF, TM, TN>(accr, acci, cr, ci);
     __syncthreads();
   }
 
@@ -188,6 +218,7 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
   constexpr int WM = BM / WGM, WN = BN / WGN;
   constexpr int TM = WM / 16, TN = WN / 16;
   static_assert(WM % 16 == 0 && WN % 16 == 0, "wave tile must be a multiple of 16");
+  constexpr bool CHUNKED = sizeof(R) == 4;
   constexpr int VEC = 16 / (int)sizeof(C);       // complex elements per 16-byte chunk
   constexpr int CPR = BKT / VEC;                  // chunks per row of a K tile
   constexpr int ACH = (BM * CPR + 255) / 256;     // chunks per lane, A tile
@@ -235,34 +266,28 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
   const int li = lane % 16, lk = lane / 16;
 
   typename MF::acc accr[TM][TN], acci[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        accr[i][j][r] = 0;
-        acci[i][j][r] = 0;
-      }
+  zero_acc<MF, TM, TN>(accr, acci);
 
-  float4 ra[ACH], rb[BCH];
+  // 16-byte staging registers: two complex64 or one complex128 per chunk
+  typedef typename std::conditional<sizeof(C) == 8, float4, double2>::type V16;
+  V16 ra[ACH], rb[BCH];
   auto load_tile = [&](uint32_t kt) {
 #pragma unroll
     for (int q = 0; q < ACH; ++q) {
       const int id = tid + 256 * q;
       const int row = id / CPR, cc = id % CPR;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      V16 v = {};
       if (row < BM && m0 + row < Mt)
-        v = *reinterpret_cast<const float4*>(A + offAm[row] + offAk[kt] + cc * VEC);
+        v = *reinterpret_cast<const V16*>(A + offAm[row] + offAk[kt] + cc * VEC);
       ra[q] = v;
     }
 #pragma unroll
     for (int q = 0; q < BCH; ++q) {
       const int id = tid + 256 * q;
       const int col = id / CPR, cc = id % CPR;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      V16 v = {};
       if (col < BN && n0 + col < Nt)
-        v = *reinterpret_cast<const float4*>(B + offBn[col] + offBk[kt] + cc * VEC);
+        v = *reinterpret_cast<const V16*>(B + offBn[col] + offBk[kt] + cc * VEC);
       rb[q] = v;
     }
   };
@@ -278,9 +303,8 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
           As_r[k + 1][row] = ra[q].z;
           As_i[k + 1][row] = ra[q].w;
         } else {
-          const double2 d = *reinterpret_cast<const double2*>(&ra[q]);
-          As_r[k][row] = d.x;
-          As_i[k][row] = d.y;
+          As_r[k][row] = ra[q].x;
+          As_i[k][row] = ra[q].y;
         }
       }
     }
@@ -295,9 +319,8 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
           Bs_r[k + 1][col] = rb[q].z;
           Bs_i[k + 1][col] = rb[q].w;
         } else {
-          const double2 d = *reinterpret_cast<const double2*>(&rb[q]);
-          Bs_r[k][col] = d.x;
-          Bs_i[k][col] = d.y;
+          Bs_r[k][col] = rb[q].x;
+          Bs_i[k][col] = rb[q].y;
         }
       }
     }
@@ -305,6 +328,8 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
 
   load_tile(0);
   for (uint32_t kt = 0; kt < ntiles; ++kt) {
+    typename MF::acc cr[TM][TN], ci[TM][TN];
+    if constexpr (CHUNKED) zero_acc<MF, TM, TN>(cr, ci);
     store_tile();
     __syncthreads();
     if (kt + 1 < ntiles) load_tile(kt + 1);
@@ -325,12 +350,15 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
-          accr[i][j] = MF::op(ar[i], br[j], accr[i][j]);
-          accr[i][j] = MF::op(-ai[i], bi[j], accr[i][j]);
-          acci[i][j] = MF::op(ar[i], bi[j], acci[i][j]);
-          acci[i][j] = MF::op(ai[i], br[j], acci[i][j]);
+          typename MF::acc& xr = CHUNKED ? cr[i][j] : accr[i][j];
+          typename MF::acc& xi = CHUNKED ? ci[i][j] : acci[i][j];
+          xr = MF::op(ar[i], br[j], xr);
+          xr = MF::op(-ai[i], bi[j], xr);
+          xi = MF::op(ar[i], bi[j], xi);
+          xi = MF::op(ai[i], br[j], xi);
         }
     }
+    if constexpr (CHUNKED) add_chunk<MF, TM, TN>(accr, acci, cr, ci);
     __syncthreads();
   }
 

@@ -153,7 +153,7 @@ def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf
 SMALL_ELEMS = 1 << 16   # steps up to this size run in FP64 under precision="mixed"
 
 
-def assign_precision(slots, steps, ctx_code, precision, small_elems=SMALL_ELEMS):
+def assign_precision(slots, steps, ctx_code, precision, small_elems=SMALL_ELEMS, fp64_steps=()):
     """Element type per slot and the step list with conversions:
     [(kind, a, b, c, M, N, K)], kind 0 = contract, 1 = convert a -> c.
 
@@ -162,7 +162,8 @@ def assign_precision(slots, steps, ctx_code, precision, small_elems=SMALL_ELEMS)
     small_elems elements per slice runs in FP64 while its operands are FP64;
     every other step runs in complex64, after a conversion of FP64 operands.
     Leaves stay FP64, so a complex64 step that reads a leaf runs in FP64
-    instead (its other operand converted up)."""
+    instead (its other operand converted up).  fp64_steps: indices of
+    contraction steps that run in FP64 whatever their size."""
     n0 = len(slots)
     dt = [ctx_code] * n0
     out = []
@@ -171,10 +172,11 @@ def assign_precision(slots, steps, ctx_code, precision, small_elems=SMALL_ELEMS)
     if ctx_code != _lib.PQ_C128:
         raise ValueError('precision="mixed" needs a ComplexF64 backend (FP64 leaves)')
     slots = list(slots)
-    for sa, sb, sc, M, N, K in steps:
+    force = set(fp64_steps)
+    for i, (sa, sb, sc, M, N, K) in enumerate(steps):
         small = all(slots[x].elems <= small_elems for x in (sa, sb, sc))
         want = _lib.PQ_C128 if (small and dt[sa] == _lib.PQ_C128 and dt[sb] == _lib.PQ_C128) else _lib.PQ_C64
-        if want == _lib.PQ_C64 and (slots[sa].leaf or slots[sb].leaf):
+        if i in force or (want == _lib.PQ_C64 and (slots[sa].leaf or slots[sb].leaf)):
             want = _lib.PQ_C128
         ops = []
         for x in (sa, sb):
@@ -275,12 +277,12 @@ def choose_group_bits(slots, n_sliced, budget_bytes, esize):
 
 
 def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=None, tiles=None,
-                 budget_bytes=64 << 30, precision="uniform"):
+                 budget_bytes=64 << 30, precision="uniform", fp64_steps=()):
     """Compile a pair plan of a network whose backend is a HIPBackend.
 
     precision: "uniform" (every step in the backend's type) or "mixed" (FP64
     backend: small steps in FP64, large ones in complex64 on FP32 MFMA; see
-    assign_precision).  kc / tiles, when given, are per contraction step of
+    assign_precision; fp64_steps forces contraction steps to FP64).  kc / tiles, when given, are per contraction step of
     `plan` (kernel class and GEMM tile, -1 = automatic)."""
     backend = network.backend
     ctx = backend.ctx
@@ -290,7 +292,7 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
         raise KeyError(f"leaf tensors missing on the device: {missing[:4]}")
     slots, steps, out_slot = lower_plan(network, plan, sliced, out_order, handles, strides)
     n_sliced = len(list(sliced))
-    dts, steps, slots = assign_precision(slots, steps, ctx.code, precision)
+    dts, steps, slots = assign_precision(slots, steps, ctx.code, precision, fp64_steps=fp64_steps)
     if group_bits is None:
         esize = 8 if precision == "mixed" or ctx.code == _lib.PQ_C64 else 16
         group_bits = choose_group_bits(slots, n_sliced, budget_bytes, esize)
