@@ -164,17 +164,20 @@ def cpu_baseline(plan, sliced, cap_s=20.0):
 TRAFFIC_FILE = ROOT / "profiles" / "gemm_grid6x6d14_traffic.json"
 
 
-def measured_traffic(roof, d):
-    """HBM bytes per launch of the dominant kernel class from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by the copy-kernel
-    calibration), or None when no such measurement is committed."""
+def measured_traffic(kernel_class, d):
+    """(HBM bytes per launch, detail) of the dominant kernel class from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by the
+    copy-kernel calibration), or (None, None) when none is committed."""
     if not TRAFFIC_FILE.exists():
-        return None
+        return None, None
     t = json.loads(TRAFFIC_FILE.read_text())
-    if t.get("kernel_class") != roof.get("kernel_class"):
-        return None
-    return {"hbm_bytes_per_run": t["hbm_bytes_per_run"], "alg_bytes_per_run": d["bytes"],
-            "ratio": t["hbm_bytes_per_run"] / d["bytes"], "source": TRAFFIC_FILE.name}
+    if t.get("kernel_class") != kernel_class:
+        return None, None
+    per_launch = t["hbm_bytes_per_run"] / t["gemm_launches_per_run"]
+    alg_per_launch = d["bytes"] / d["steps"]
+    return per_launch, {"hbm_bytes_per_run": t["hbm_bytes_per_run"], "alg_bytes_per_run": d["bytes"],
+                        "hbm_over_alg": t["hbm_bytes_per_run"] / d["bytes"],
+                        "alg_bytes_per_launch": alg_per_launch, "source": "profiles/" + TRAFFIC_FILE.name}
 
 
 def calibrate_copy(mib):
@@ -277,10 +280,20 @@ def main():
         achieved = d["bytes"] / (d["ms"] * 1e-3)
         roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK["hbm"] / 1e9,
                 "unit": "GB/s", "frac": achieved / PEAK["hbm"]}
-    roof["traffic"] = measured_traffic(roof, d)
+    roof["traffic"], roof["traffic_detail"] = measured_traffic(_lib.KC_NAMES[dom], d)
     roof["kernel_class"] = _lib.KC_NAMES[dom]
+    roof["launches_per_run"] = d["steps"]
     roof["class_ms_per_run"] = d["ms"]
     roof["classes"] = {_lib.KC_NAMES[k]: round(v["ms"], 3) for k, v in per.items()}
+    # Whole-plan roofline: each step's lower bound max(flops / MFMA peak of
+    # its element type, bytes / HBM peak), summed, against the measured run.
+    lb = 0.0
+    for i, (kind, sa, sb, sc, M, N, K) in enumerate(p.steps):
+        inf = p.step_info(i)
+        peak = PEAK["mfma_fp64"] if p.dtypes[sc] == _lib.PQ_C128 else PEAK["mfma_fp32"]
+        lb += max(inf["flops"] / peak, inf["bytes"] / PEAK["hbm"])
+    plan_roof = {"roofline_ms": lb * 1e3, "events_ms": float(sum(ms)), "wall_ms": 1e3 * dt / args.steps,
+                 "frac_of_wall": lb * 1e3 / (1e3 * dt / args.steps)}
     p.close()
 
     # What the learned model buys: the pick FLOP counting alone would make,
@@ -335,13 +348,15 @@ def main():
         "metric": METRIC, "value": amps, "unit": "amplitudes/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "c64 on fp32 MFMA (sweep); fp64 leaves, world lines and slice sum",
+        "dtype": "c64",
+        "precision": "complex64 on FP32 MFMA for the large steps; FP64 leaves, small steps and slice sum",
         "data": "synthetic",
         "config": {"workload": "6x6 grid depth-14 random circuit (seed 2026), 10 sliced bonds = 1024 slices, "
                                "64 bitstrings (last column open), one hipGraph per run",
                    "qubits": 36, "depth": DEPTH, "slices": 2 ** N_SLICED, "bitstrings": 64,
                    "parallelism": "single GPU"},
         "roofline": roof,
+        "plan_roofline": plan_roof,
         "cpu_baseline": cpu,
         "rel_err_vs_fp64": rel_err,
         "rel_err_vs_fp64_normwise": rel_err_norm,

@@ -112,8 +112,15 @@ int pq_set_kernel_override(pq_ctx* ctx, int kclass);
 
 /* GEMM workgroup tile override (-1 = automatic): 0:32x32 1:64x32 2:32x64 3:64x64
  * 4:128x64 5:64x128 6:64x16 7:128x16 8:256x16 9:16x64 10:128x32 11:128x128
+ * 12:16x128 13:16x256 14:32x128, rows x columns of the product as launched
  * (complex128 tiles are capped at 4096 elements and 128 rows or columns). */
+#define PQ_GEMM_TILES 15
 int pq_set_gemm_tile(pq_ctx* ctx, int tile);
+
+/* GEMM orientation override: -1 = automatic (the product is launched
+ * transposed, C^T = B^T A^T, when C's unit stride lies on A's open side, so
+ * that adjacent lanes store adjacent outputs), 0 = as given, 1 = transposed. */
+int pq_set_gemm_orientation(pq_ctx* ctx, int mode);
 
 /* ------------------------------------------------------------------------
  * Planned execution: a whole pair plan (contract_network!, src/layer2.jl:

@@ -448,6 +448,9 @@ hipError_t launch_gemm(const C* A, const C* B, C* Out, const GemmArgs& g, hipStr
     case 9: return launch_tile<C, 16, 64, 1>(A, B, Out, g, s);
     case 10: return launch_tile<C, 128, 32, 4>(A, B, Out, g, s);
     case 11: return launch_tile<C, 128, 128, 2>(A, B, Out, g, s);
+    case 12: return launch_tile<C, 16, 128, 1>(A, B, Out, g, s);
+    case 13: return launch_tile<C, 16, 256, 1>(A, B, Out, g, s);
+    case 14: return launch_tile<C, 32, 128, 1>(A, B, Out, g, s);
     default: return hipErrorInvalidValue;
   }
 }

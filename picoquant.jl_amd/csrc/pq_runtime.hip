@@ -206,7 +206,7 @@ int materialize_into(pq_ctx* c, const Tensor& src, Tensor& dst) {
 // fill them, narrow ones otherwise (complex128 is limited to 64x64).
 static int pick_tile(uint64_t Mt, uint64_t Nt, size_t esize) {
   if (Nt <= 16) return Mt >= 4096 && esize == 8 ? 8 : (Mt >= 128 ? 7 : 6);
-  if (Mt <= 16) return 9;
+  if (Mt <= 16) return Nt >= 4096 && esize == 8 ? 13 : (Nt >= 128 ? 12 : 9);
   if (Mt <= 32 && Nt <= 32) return 0;
   if (Nt <= 32) return Mt >= 128 ? 10 : 1;
   if (Mt <= 32) return 2;
@@ -246,8 +246,16 @@ int resolve_step(pq_ctx* c, StepSpec& s, size_t esize) {
     s.stream_ab = ab && (Mt >= Nt || !ba);
   } else if (kc == PQ_KC_GEMM) {
     if (!gemm_ok) return fail(c, PQ_ERR_UNSUPPORTED, "gemm kernel not applicable");
+    // The MFMA result layout puts 16 consecutive lanes on 16 consecutive
+    // N-side outputs; when C's unit stride lies on the M side the product
+    // is computed transposed so that those lanes write adjacent addresses.
+    bool c_unit_m = false, c_unit_n = false;
+    for (auto& m : s.M) c_unit_m = c_unit_m || m.s1 == 1;
+    for (auto& m : s.N) c_unit_n = c_unit_n || m.s1 == 1;
+    s.gemm_swap = c->gemm_orient >= 0 ? c->gemm_orient == 1 : (c_unit_m && !c_unit_n);
     int t = s.tile >= 0 ? s.tile : c->gemm_tile;
-    s.tile = t >= 0 ? t : pick_tile(Mt, Nt, esize);
+    if (t >= PQ_GEMM_TILES) return fail(c, PQ_ERR_ARG, "GEMM tile out of range");
+    s.tile = t >= 0 ? t : (s.gemm_swap ? pick_tile(Nt, Mt, esize) : pick_tile(Mt, Nt, esize));
   } else if (kc == PQ_KC_REDUCE) {
     if (Mt * Nt > 16) return fail(c, PQ_ERR_UNSUPPORTED, "reduce kernel needs M*N <= 16");
   } else if (kc != PQ_KC_GENERIC) {
@@ -290,12 +298,25 @@ static int launch_typed(pq_ctx* c, const C* pa, const C* pb, C* pc, const StepSp
       e = launch_stream<C>(pb, pa, pc, sa, c->stream);
     }
   } else if (s.kc == PQ_KC_GEMM) {
+    // Operand roles as launched: transposed product swaps A/B, M/N, the K
+    // strides and the batch strides.
+    const std::vector<Mode>& gM = s.gemm_swap ? s.N : s.M;
+    const std::vector<Mode>& gN = s.gemm_swap ? s.M : s.N;
+    std::vector<Mode> gK = s.K;
+    Batch L = s.L;
+    const C* xa = pa;
+    const C* xb = pb;
+    if (s.gemm_swap) {
+      for (auto& m : gK) std::swap(m.s0, m.s1);
+      for (int j = 0; j < L.nmodes; ++j) std::swap(L.sA[j], L.sB[j]);
+      std::swap(xa, xb);
+    }
     GemmArgs ga;
     memset(&ga, 0, sizeof(ga));
-    if (!fill_group(ga.M, s.M) || !fill_group(ga.N, s.N) || !fill_group(ga.K, s.K))
+    if (!fill_group(ga.M, gM) || !fill_group(ga.N, gN) || !fill_group(ga.K, gK))
       return fail(c, PQ_ERR_UNSUPPORTED, "too many modes");
-    ga.L = s.L;
-    for (auto& m : s.K) {
+    ga.L = L;
+    for (auto& m : gK) {
       if (m.s0 == 1) ga.a_kfast = 1;
       if (m.s1 == 1) ga.b_kfast = 1;
     }
@@ -304,16 +325,16 @@ static int launch_typed(pq_ctx* c, const C* pa, const C* pb, C* pc, const StepSp
     // operands and every element offset keeps 16-byte alignment.
     {
       const int64_t vec = 16 / (int64_t)sizeof(C);
-      bool ok = !s.K.empty() && s.K[0].s0 == 1 && s.K[0].s1 == 1 && s.K[0].ext % 8 == 0 &&
-                ((uintptr_t)pa % 16) == 0 && ((uintptr_t)pb % 16) == 0;
-      for (size_t q = 1; q < s.K.size() && ok; ++q) ok = s.K[q].s0 % vec == 0 && s.K[q].s1 % vec == 0;
-      for (auto& m : s.M) ok = ok && m.s0 % vec == 0;
-      for (auto& m : s.N) ok = ok && m.s0 % vec == 0;
-      for (int j = 0; j < s.L.nmodes && ok; ++j) ok = s.L.sA[j] % vec == 0 && s.L.sB[j] % vec == 0;
+      bool ok = !gK.empty() && gK[0].s0 == 1 && gK[0].s1 == 1 && gK[0].ext % 8 == 0 &&
+                ((uintptr_t)xa % 16) == 0 && ((uintptr_t)xb % 16) == 0;
+      for (size_t q = 1; q < gK.size() && ok; ++q) ok = gK[q].s0 % vec == 0 && gK[q].s1 % vec == 0;
+      for (auto& m : gM) ok = ok && m.s0 % vec == 0;
+      for (auto& m : gN) ok = ok && m.s0 % vec == 0;
+      for (int j = 0; j < L.nmodes && ok; ++j) ok = L.sA[j] % vec == 0 && L.sB[j] % vec == 0;
       ga.contig = ok ? 1 : 0;
     }
     ProfScope ps(c, PQ_KC_GEMM, alg_bytes, alg_flops);
-    e = launch_gemm<C>(pa, pb, pc, ga, c->stream);
+    e = launch_gemm<C>(xa, xb, pc, ga, c->stream);
   } else {
     GenArgs ga;
     memset(&ga, 0, sizeof(ga));
@@ -700,8 +721,14 @@ int pq_profile_read(pq_ctx* c, int kc, int64_t* launches, double* total_ms, doub
 }
 
 int pq_set_gemm_tile(pq_ctx* c, int tile) {
-  if (!c || tile < -1 || tile > 11) return PQ_ERR_ARG;
+  if (!c || tile < -1 || tile >= PQ_GEMM_TILES) return PQ_ERR_ARG;
   c->gemm_tile = tile;
+  return PQ_OK;
+}
+
+int pq_set_gemm_orientation(pq_ctx* c, int mode) {
+  if (!c || mode < -1 || mode > 1) return PQ_ERR_ARG;
+  c->gemm_orient = mode;
   return PQ_OK;
 }
 

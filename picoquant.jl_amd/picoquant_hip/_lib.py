@@ -22,6 +22,7 @@ PQ_ERR_UNSUPPORTED = 5
 PQ_C128 = 0
 PQ_C64 = 1
 PQ_MAX_RANK = 64
+PQ_GEMM_TILES = 15
 
 KC_STREAM, KC_GEMM, KC_GENERIC, KC_REDUCE, KC_COPY, KC_BATCHED, KC_ACCUM = range(7)
 KC_NAMES = {KC_STREAM: "stream", KC_GEMM: "gemm", KC_GENERIC: "generic", KC_REDUCE: "reduce",
@@ -53,6 +54,7 @@ SIGNATURES = {
                                    ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
     "pq_set_kernel_override": (ct.c_int, [_P, ct.c_int]),
     "pq_set_gemm_tile": (ct.c_int, [_P, ct.c_int]),
+    "pq_set_gemm_orientation": (ct.c_int, [_P, ct.c_int]),
     "pq_plan_create": (ct.c_int, [_P, _P, ct.POINTER(_P)]),
     "pq_plan_execute": (ct.c_int, [_P, _P, ct.c_int]),
     "pq_plan_result": (ct.c_int, [_P, _P, ct.POINTER(ct.c_double)]),
@@ -210,6 +212,10 @@ class Context:
 
     def set_gemm_tile(self, tile):
         self.check(self.lib.pq_set_gemm_tile(self.ptr, int(tile)))
+
+    def set_gemm_orientation(self, mode):
+        """-1 automatic, 0 as given, 1 transposed (C^T = B^T A^T)."""
+        self.check(self.lib.pq_set_gemm_orientation(self.ptr, int(mode)))
 
     def time_step(self, M, N, K, batch=1, kclass=-1, tile=-1, reps=3, dtype=None):
         """Median microseconds of one contraction shape (modes as (ext, s0, s1))."""

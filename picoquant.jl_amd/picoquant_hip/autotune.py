@@ -12,7 +12,7 @@ from . import _lib
 from .costmodel import merge
 from .executor import SMALL_ELEMS, assign_precision, lower_plan
 
-TILES = {_lib.PQ_C64: list(range(12)), _lib.PQ_C128: list(range(11))}
+TILES = {_lib.PQ_C64: list(range(15)), _lib.PQ_C128: list(range(11)) + [12, 14]}
 
 
 def tune_step(ctx, M, N, K, batch, dtype, reps=3, cache=None):

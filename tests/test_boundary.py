@@ -33,3 +33,18 @@ def test_load_binds_signatures():
     lib = _lib.load()
     for name in _lib.SIGNATURES:
         assert getattr(lib, name).argtypes is not None
+
+
+def test_python_constants_match_header():
+    txt = (ROOT / "include" / "pq_hip.h").read_text()
+    defs = {k: int(v) for k, v in re.findall(r"#define\s+(PQ_[A-Z0-9_]+)\s+(\d+)", txt)}
+    kc = {"PQ_KC_" + n.upper(): v for v, n in _lib.KC_NAMES.items()}
+    checked = 0
+    for name, val in defs.items():
+        if hasattr(_lib, name):
+            assert getattr(_lib, name) == val, name
+            checked += 1
+        elif name in kc:
+            assert kc[name] == val, name
+            checked += 1
+    assert checked >= 10

@@ -17,24 +17,28 @@ def ctx(request):
     c.close()
 
 
-def gemm(ctx, A, ia, B, ib, tile=-1):
+def gemm(ctx, A, ia, B, ib, tile=-1, orient=-1):
     ha, hb = ctx.upload(A), ctx.upload(B)
     ctx.set_kernel_override(_lib.KC_GEMM)
     ctx.set_gemm_tile(tile)
+    ctx.set_gemm_orientation(orient)
     try:
         hc = ctx.contract(ha, ia, hb, ib)
     finally:
         ctx.set_kernel_override(-1)
         ctx.set_gemm_tile(-1)
+        ctx.set_gemm_orientation(-1)
     out = ctx.download(hc)
     for h in (ha, hb, hc):
         ctx.free(h)
     return out
 
 
-@pytest.mark.parametrize("tile", list(range(12)))
-def test_tiles_and_access_orders(ctx, tile):
-    """C[m,n] = A[m,k] B[k,n] with K fastest / slowest in each operand."""
+@pytest.mark.parametrize("orient", [0, 1], ids=["as-given", "transposed"])
+@pytest.mark.parametrize("tile", list(range(_lib.PQ_GEMM_TILES)))
+def test_tiles_and_access_orders(ctx, tile, orient):
+    """C[m,n] = A[m,k] B[k,n] with K fastest / slowest in each operand, the
+    product launched as given and transposed (C^T = B^T A^T)."""
     rng = np.random.default_rng(tile)
     M, N, K = 80, 72, 40          # ragged against every tile
     for a_kfast in (False, True):
@@ -43,7 +47,7 @@ def test_tiles_and_access_orders(ctx, tile):
             ia = [1, -1] if a_kfast else [-1, 1]
             B = rand(rng, (K, N)) if b_kfast else rand(rng, (N, K))
             ib = [1, -2] if b_kfast else [-2, 1]
-            check(ctx, A, ia, B, ib, gemm(ctx, A, ia, B, ib, tile))
+            check(ctx, A, ia, B, ib, gemm(ctx, A, ia, B, ib, tile, orient))
 
 
 def test_multimode_groups(ctx):
