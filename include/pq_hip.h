@@ -144,7 +144,10 @@ typedef struct {
   const int32_t* slot_batched;      /* 1 if the slot depends on a sliced bond */
   const pq_tensor* slot_leaf;       /* leaf tensor, 0 for intermediates */
   int32_t n_sliced;                 /* number of sliced bonds (extent 2 each) */
-  const int64_t* leaf_slice_stride; /* [n_slots * n_sliced] stride of bond j in a leaf, 0 if absent */
+  const int64_t* slot_slice_stride; /* [n_slots * n_sliced] stride of sliced bond j in an unbatched
+                                       slot that keeps it as an index (a leaf, or an intermediate
+                                       computed for both values of the bond), 0 if absent; batched
+                                       steps read such a slot at each slice's offset */
   int32_t group_bits;               /* slices per launch = 2^group_bits (bonds 0..group_bits-1) */
   int32_t n_steps;
   const int32_t* step_slots;        /* [n_steps * 3] A, B, C slots */

@@ -13,8 +13,10 @@
 //  * Slices: each sliced bond has extent 2.  Slice s fixes bond j to bit j
 //    of s.  Steps whose operands do not depend on any sliced bond run once
 //    (phase 1); the others (phase 2) run once per slice group, batched over
-//    the 2^group_bits slices of the group on grid z, reading leaves through
-//    per-bond strides and batched intermediates through entry strides.
+//    the 2^group_bits slices of the group on grid z, reading batched
+//    intermediates through entry strides and slots that keep sliced bonds as
+//    indices (leaves, and small intermediates computed once for both values
+//    of their bonds) through per-bond strides.
 //  * Small steps of one dependency level and element type share one grouped
 //    launch (the world lines of all qubits advance together).
 //  * The per-slice result is summed over slices in FP64, deterministic in
@@ -67,7 +69,7 @@ struct Slot {
   bool batched = false;
   int dtype = PQ_C128;
   pq_tensor leaf = 0;
-  std::vector<int64_t> slice_stride;  // leaves: per sliced bond
+  std::vector<int64_t> slice_stride;  // unbatched slots: stride of each sliced bond kept (0 = absent)
   int64_t arena_off = -1;             // intermediates (bytes)
   int producer = -1;
   int last_use = -1;
@@ -120,16 +122,18 @@ struct pq_plan {
 
 namespace {
 
+// Address of slot s for slice group `group`: slots that keep sliced bonds as
+// indices are offset by the group's high bits (phase 1 runs with group 0).
 void* slot_ptr(pq_ctx* c, pq_plan* p, int s, int64_t group) {
   Slot& sl = p->slots[s];
+  int64_t off = 0;
+  for (int j = p->group_bits; j < p->n_sliced; ++j)
+    if ((group >> (j - p->group_bits)) & 1) off += sl.slice_stride[j];
   if (sl.leaf) {
     Tensor* t = get(c, sl.leaf);
-    int64_t off = t->off;
-    for (int j = p->group_bits; j < p->n_sliced; ++j)
-      if ((group >> (j - p->group_bits)) & 1) off += sl.slice_stride[j];
-    return (unsigned char*)t->buf->ptr + off * c->esize;
+    return (unsigned char*)t->buf->ptr + (t->off + off) * c->esize;
   }
-  return (unsigned char*)p->arena + sl.arena_off;
+  return (unsigned char*)p->arena + sl.arena_off + off * (int64_t)esize_of(sl.dtype);
 }
 
 int64_t span_of(const std::vector<Mode>& ms, bool first) {
@@ -262,6 +266,15 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     sl.dtype = d->slot_dtype ? d->slot_dtype[s] : c->dtype;
     if (sl.dtype != PQ_C128 && sl.dtype != PQ_C64) return bad("bad slot element type");
     if (sl.elems < 1 || sl.elems >= (1ll << 31)) return bad("slot size out of range");
+    sl.slice_stride.assign(d->n_sliced, 0);
+    int64_t extra = 0;
+    for (int j = 0; j < d->n_sliced; ++j) {
+      const int64_t v = d->slot_slice_stride ? d->slot_slice_stride[(int64_t)s * d->n_sliced + j] : 0;
+      if (v < 0) return bad("negative slice stride");
+      if (v && sl.batched) return bad("a batched slot cannot keep sliced bonds");
+      sl.slice_stride[j] = v;
+      extra += v;
+    }
     if (sl.leaf) {
       Tensor* t = get(c, sl.leaf);
       if (!t) return bad("unknown leaf tensor");
@@ -270,8 +283,9 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
       int64_t span = 0;
       for (int i = 0; i < t->rank; ++i) span += (t->dims[i] - 1) * t->strides[i];
       if (span + 1 > (int64_t)(t->buf->bytes / c->esize) - t->off) return bad("leaf view exceeds its buffer");
-      sl.slice_stride.assign(d->n_sliced, 0);
-      for (int j = 0; j < d->n_sliced; ++j) sl.slice_stride[j] = d->leaf_slice_stride[(int64_t)s * d->n_sliced + j];
+      if (extra > span) return bad("slice strides exceed the leaf");
+    } else if (extra >= sl.elems) {
+      return bad("slice strides exceed the slot");
     }
   }
   // Steps.
@@ -314,7 +328,8 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     st.dep = Cs.batched;
     if (!st.dep && (A.batched || B.batched)) return bad("unbatched output from a batched operand");
     if (st.kind == 1) {
-      if (A.leaf || A.elems != Cs.elems || A.batched != Cs.batched) return bad("convert needs a same-shaped intermediate");
+      if (A.leaf || A.elems != Cs.elems || A.batched != Cs.batched || A.slice_stride != Cs.slice_stride)
+        return bad("convert needs a same-shaped intermediate");
       const double n = (double)Cs.elems * (Cs.batched ? (double)G * p->n_groups : 1.0);
       st.bytes = n * (esize_of(A.dtype) + esize_of(Cs.dtype));
       p->alg_bytes += st.bytes;
@@ -323,14 +338,16 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     }
     if (A.dtype != Cs.dtype || B.dtype != Cs.dtype) return bad("contraction operands differ in element type");
     st.spec.dtype = Cs.dtype;
-    // In-bounds check of every operand for one batch entry; a leaf must also
-    // hold every slice offset on top of the access span.
+    // In-bounds check of every operand for one batch entry; a batched step
+    // also reads slots that keep sliced bonds at every slice offset.
     auto fits = [&](const Slot& sl, int64_t span) -> bool {
-      if (!sl.leaf) return span < sl.elems;
+      int64_t extra = 0;
+      if (st.dep)
+        for (int64_t v : sl.slice_stride) extra += v;
+      if (!sl.leaf) return span + extra < sl.elems;
       Tensor* t = get(c, sl.leaf);
-      int64_t full = 0, extra = 0;
+      int64_t full = 0;
       for (int q = 0; q < t->rank; ++q) full += (t->dims[q] - 1) * t->strides[q];
-      for (int64_t v : sl.slice_stride) extra += v;
       return span + extra <= full;
     };
     if (!fits(A, span_of(st.spec.M, true) + span_of(st.spec.K, true))) return bad("A access out of range");
@@ -343,8 +360,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
       L.nmodes = d->group_bits;
       L.total = (uint32_t)G;
       auto stride_of = [&](const Slot& sl, int j) -> int64_t {
-        if (sl.leaf) return sl.slice_stride[j];
-        return sl.batched ? sl.elems * ((int64_t)1 << j) : 0;
+        return sl.batched ? sl.elems * ((int64_t)1 << j) : sl.slice_stride[j];
       };
       for (int j = 0; j < d->group_bits; ++j) {
         L.ext[j] = make_fdiv(2);
@@ -372,19 +388,6 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
   }
   if (!p->slots[p->out_slot].leaf && produced[p->out_slot] < 0) return bad("output slot never produced");
   if (d->n_sliced > 0 && !p->slots[p->out_slot].batched) return bad("sliced plan with a slice-independent output");
-  // Leaf bounds over all slice offsets.
-  for (auto& sl : p->slots) {
-    if (!sl.leaf) continue;
-    Tensor* t = get(c, sl.leaf);
-    int64_t extra = 0;
-    for (int64_t v : sl.slice_stride) {
-      if (v < 0) return bad("negative slice stride");
-      extra += v;
-    }
-    int64_t span = 0;
-    for (int i = 0; i < t->rank; ++i) span += (t->dims[i] - 1) * t->strides[i];
-    if (extra > span) return bad("slice strides exceed the leaf");
-  }
   // Launch sequences: by dependency level; the small contraction steps of a
   // level and element type share one grouped launch, the others launch alone.
   auto small = [&](const PStep& st) {
@@ -523,11 +526,11 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
           return fail(c, PQ_ERR_UNSUPPORTED, "too many modes in a grouped step");
         }
         m.L = st.spec.L;
-        m.nhigh = d->n_sliced - d->group_bits;
+        m.nhigh = st.dep ? d->n_sliced - d->group_bits : 0;
         for (int j = 0; j < m.nhigh; ++j) {
           const int bit = d->group_bits + j;
-          m.hA[j] = p->slots[st.a].leaf ? p->slots[st.a].slice_stride[bit] : 0;
-          m.hB[j] = p->slots[st.b].leaf ? p->slots[st.b].slice_stride[bit] : 0;
+          m.hA[j] = p->slots[st.a].slice_stride[bit];
+          m.hB[j] = p->slots[st.b].slice_stride[bit];
           m.hC[j] = 0;
         }
         hd.push_back(m);

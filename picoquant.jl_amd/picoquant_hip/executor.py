@@ -4,6 +4,10 @@ libpq_hip's plan executor (pq_plan_* in include/pq_hip.h).
 compile_plan() lowers (network, pair plan, sliced bonds) to slots and steps:
   * leaves are the network's device tensors; a sliced bond stays in the leaf
     and is selected per slice through its stride;
+  * small intermediates keep the sliced bonds of their operands as ordinary
+    indices (computed once, for both values of each bond) when that keeps
+    them small; the first large step that reads them selects the slice
+    through the bond's stride, as for leaves;
   * an intermediate is laid out with the indices its consumer sums first
     (so GEMM operand loads run along K), the rest after; the final tensor is
     laid out in output-qubit order, which is the reference's final permute
@@ -29,7 +33,7 @@ class PlanDesc(ct.Structure):
         ("slot_batched", ct.POINTER(ct.c_int32)),
         ("slot_leaf", ct.POINTER(ct.c_int64)),
         ("n_sliced", ct.c_int32),
-        ("leaf_slice_stride", ct.POINTER(ct.c_int64)),
+        ("slot_slice_stride", ct.POINTER(ct.c_int64)),
         ("group_bits", ct.c_int32),
         ("n_steps", ct.c_int32),
         ("step_slots", ct.POINTER(ct.c_int32)),
@@ -47,24 +51,36 @@ def _arr(ctype, xs):
     return (ctype * max(1, len(xs)))(*xs)
 
 
+SMALL_ELEMS = 1 << 16   # steps up to this size run in FP64 under precision="mixed"
+
+
 class SlotInfo:
-    __slots__ = ("idx", "ext", "stride", "leaf", "dep", "slice_stride", "elems")
+    __slots__ = ("idx", "ext", "stride", "leaf", "dep", "slice_stride", "carry", "elems")
 
     def __init__(self):
         self.idx = []          # per-slice indices, in layout order
         self.ext = {}          # index -> extent
         self.stride = {}       # index -> per-slice stride
         self.leaf = 0
-        self.dep = False
-        self.slice_stride = []
+        self.dep = False       # batched over the slices of a slice group
+        self.slice_stride = [] # per sliced bond: its stride when kept as an index, else 0
+        self.carry = []        # intermediates: sliced bonds kept as indices (laid out last)
         self.elems = 1
 
 
-def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf_strides=None):
+def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf_strides=None,
+               carry_elems=SMALL_ELEMS):
     """Pure-host lowering (no device calls): returns (slots, steps, out_slot)
-    where steps = [(a, b, c, M, N, K)] with modes as (ext, s0, s1)."""
+    where steps = [(a, b, c, M, N, K)] with modes as (ext, s0, s1).
+
+    carry_elems: an intermediate whose operands are slice-independent keeps
+    their sliced bonds as indices (runs once, for every value of those
+    bonds) when it, with them, has at most this many elements and no sliced
+    bond joins its two operands; 0 turns this off (every intermediate that
+    touches a sliced bond is then batched over the slices)."""
     sliced = list(sliced)
     sset = set(sliced)
+    nsl = len(sliced)
     slots: list[SlotInfo] = []
     label_slot = {}
     for lab, node in network.nodes.items():
@@ -81,7 +97,6 @@ def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf
                 if d != 2:
                     raise ValueError(f"sliced bond {ix} has extent {d}; only extent 2 is supported")
                 s.slice_stride[sliced.index(ix)] = st
-                s.dep = True
             else:
                 s.idx.append(ix)
                 s.ext[ix] = d
@@ -100,7 +115,17 @@ def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf
         C = SlotInfo()
         C.idx = c_idx
         C.ext = {x: (A.ext.get(x) or B.ext.get(x)) for x in c_idx}
-        C.dep = A.dep or B.dep
+        # sliced bonds each operand keeps as indices (leaves: strides are
+        # known; intermediates: laid out in pass 2)
+        ca = set(A.carry) | {j for j, v in enumerate(A.slice_stride) if v}
+        cb = set(B.carry) | {j for j, v in enumerate(B.slice_stride) if v}
+        size = math.prod(C.ext.values()) << len(ca | cb)
+        if (ca or cb) and not (A.dep or B.dep or ca & cb) and max(size, A.elems, B.elems) <= carry_elems:
+            C.carry = sorted(ca | cb)
+        else:
+            C.dep = A.dep or B.dep or bool(ca or cb)
+        C.slice_stride = [0] * nsl
+        C.elems = size if C.carry else math.prod(C.ext.values())
         counter += 1
         label_slot[f"node_{counter}"] = len(slots)
         raw.append((sa, sb, len(slots)))
@@ -108,6 +133,10 @@ def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf
     if len(label_slot) != 1:
         raise ValueError(f"plan leaves {len(label_slot)} tensors uncontracted")
     (out_slot,) = label_slot.values()
+    if slots[out_slot].carry:
+        # the result is per slice (summed over slices afterwards)
+        slots[out_slot].carry = []
+        slots[out_slot].dep = True
     # Pass 2: layouts.  Consumer of each intermediate -> indices summed there.
     consumer = {}
     for sa, sb, sc in raw:
@@ -137,20 +166,29 @@ def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf
         for x in C.idx:
             C.stride[x] = st
             st *= C.ext[x]
+        for j in C.carry:
+            C.slice_stride[j] = st
+            st *= 2
         C.elems = st
     # Pass 3: modes.
     steps = []
     for sa, sb, sc in raw:
         A, B, C = slots[sa], slots[sb], slots[sc]
         bset, aset = set(B.idx), set(A.idx)
-        M = sorted([(A.ext[x], A.stride[x], C.stride[x]) for x in A.idx if x not in bset], key=lambda m: m[2])
-        N = sorted([(B.ext[x], B.stride[x], C.stride[x]) for x in B.idx if x not in aset], key=lambda m: m[2])
+        M = [(A.ext[x], A.stride[x], C.stride[x]) for x in A.idx if x not in bset]
+        N = [(B.ext[x], B.stride[x], C.stride[x]) for x in B.idx if x not in aset]
         K = sorted([(A.ext[x], A.stride[x], B.stride[x]) for x in A.idx if x in bset], key=lambda m: m[1])
+        for j in C.carry:      # kept sliced bonds are open indices of this step
+            if A.slice_stride[j]:
+                M.append((2, A.slice_stride[j], C.slice_stride[j]))
+            else:
+                N.append((2, B.slice_stride[j], C.slice_stride[j]))
+        M.sort(key=lambda m: m[2])
+        N.sort(key=lambda m: m[2])
         steps.append((sa, sb, sc, M, N, K))
     return slots, steps, out_slot
 
 
-SMALL_ELEMS = 1 << 16   # steps up to this size run in FP64 under precision="mixed"
 
 
 def assign_precision(slots, steps, ctx_code, precision, small_elems=SMALL_ELEMS, fp64_steps=()):
@@ -185,6 +223,7 @@ def assign_precision(slots, steps, ctx_code, precision, small_elems=SMALL_ELEMS,
                 y = SlotInfo()
                 y.idx, y.ext, y.stride = list(src.idx), dict(src.ext), dict(src.stride)
                 y.dep, y.elems = src.dep, src.elems
+                y.slice_stride, y.carry = list(src.slice_stride), list(src.carry)
                 slots.append(y)
                 dt.append(want)
                 out.append((1, x, x, len(slots) - 1, [], [], []))
@@ -277,7 +316,7 @@ def choose_group_bits(slots, n_sliced, budget_bytes, esize):
 
 
 def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=None, tiles=None,
-                 budget_bytes=64 << 30, precision="uniform", fp64_steps=()):
+                 budget_bytes=64 << 30, precision="uniform", fp64_steps=(), carry_elems=SMALL_ELEMS):
     """Compile a pair plan of a network whose backend is a HIPBackend.
 
     precision: "uniform" (every step in the backend's type) or "mixed" (FP64
@@ -290,7 +329,7 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
     missing = [lab for lab in network.nodes if lab not in handles]
     if missing:
         raise KeyError(f"leaf tensors missing on the device: {missing[:4]}")
-    slots, steps, out_slot = lower_plan(network, plan, sliced, out_order, handles, strides)
+    slots, steps, out_slot = lower_plan(network, plan, sliced, out_order, handles, strides, carry_elems)
     n_sliced = len(list(sliced))
     dts, steps, slots = assign_precision(slots, steps, ctx.code, precision, fp64_steps=fp64_steps)
     if group_bits is None:
@@ -301,7 +340,7 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
     slot_leaf = [s.leaf for s in slots]
     lss = []
     for s in slots:
-        lss += (s.slice_stride if s.leaf else [0] * n_sliced)
+        lss += (list(s.slice_stride) if s.slice_stride else [0] * n_sliced)
     step_slots, nmodes, modes, kinds = [], [], [], []
     kcs_in = list(kc) if kc is not None else None
     tls_in = list(tiles) if tiles is not None else None

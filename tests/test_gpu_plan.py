@@ -54,15 +54,21 @@ def rel(got, want):
     return float(np.max(np.abs(g - np.asarray(want).reshape(-1, order="F")) / np.abs(want).reshape(-1, order="F")))
 
 
+@pytest.mark.parametrize("carry", [True, False], ids=["carry", "no-carry"])
 @pytest.mark.parametrize("nslice,group_bits", [(0, 0), (4, 0), (4, 2), (4, 4), (6, 3)])
-def test_grid3x3_sliced(dtype, nslice, group_bits):
+def test_grid3x3_sliced(dtype, nslice, group_bits, carry):
+    """Slice groups of every size; with small intermediates keeping their
+    sliced bonds (read at per-slice and per-group offsets) and without."""
     cfg = "1x0x11x01"
     tn = grid_net(3, 3, 6, 21, cfg, dtype)
     sv = O.statevector(9, O.grid_circuit_gates(3, 3, 6, 21))
     open_q = [i + 1 for i, ch in enumerate(cfg) if ch == "x"]
     plan = PL.sweep_plan(tn, PL.grid_site_order(3, 3, open_last=open_q))
     sliced = PL.choose_slices(tn, plan, nslice)[0] if nslice else []
-    p = compile_plan(tn, plan, sliced, group_bits=group_bits)
+    kw = {} if carry else {"carry_elems": 0}
+    p = compile_plan(tn, plan, sliced, group_bits=group_bits, **kw)
+    if carry and nslice:
+        assert any(s.carry for s in p.slots)
     want = batch_amps(sv, cfg)
     for use_graph in (False, True, True, True):
         p.execute(use_graph)

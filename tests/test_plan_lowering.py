@@ -25,25 +25,28 @@ def offsets(modes, which):
     return offs
 
 
-def interpret(network, leaf_data, plan, sliced):
-    slots, steps, out_slot = lower_plan(network, plan, sliced)
+def interpret(network, leaf_data, plan, sliced, carry_elems=None):
+    kw = {} if carry_elems is None else {"carry_elems": carry_elems}
+    slots, steps, out_slot = lower_plan(network, plan, sliced, **kw)
     nslices = 2 ** len(sliced)
     total = None
     for s in range(nslices):
         bits = [(s >> j) & 1 for j in range(len(sliced))]
-        mem = {}
-        for i, sl in enumerate(slots):
-            if i < len(network.nodes):
-                flat = leaf_data[i]
-                mem[i] = (flat, sum(b * st for b, st in zip(bits, sl.slice_stride)))
+        mem = {i: leaf_data[i] for i in range(len(network.nodes))}
         for sa, sb, sc, M, N, K in steps:
-            (fa, oa), (fb, ob) = mem[sa], mem[sb]
-            Am = fa[oa + offsets(M, 0)[:, None] + offsets(K, 0)[None, :]]
-            Bm = fb[ob + offsets(K, 1)[:, None] + offsets(N, 0)[None, :]]
+            # a step batched over slices reads slots that keep sliced bonds
+            # (leaves, carried intermediates) at this slice's offset; a
+            # carrying step has those bonds among its modes instead
+            def off(x):
+                if not slots[sc].dep:
+                    return 0
+                return sum(b * st for b, st in zip(bits, slots[x].slice_stride))
+            Am = mem[sa][off(sa) + offsets(M, 0)[:, None] + offsets(K, 0)[None, :]]
+            Bm = mem[sb][off(sb) + offsets(K, 1)[:, None] + offsets(N, 0)[None, :]]
             out = np.zeros(slots[sc].elems, dtype=np.complex128)
             out[offsets(M, 1)[:, None] + offsets(N, 1)[None, :]] = Am @ Bm
-            mem[sc] = (out, 0)
-        res = mem[out_slot][0]
+            mem[sc] = out
+        res = mem[out_slot]
         total = res if total is None else total + res
     return total, slots[out_slot]
 
@@ -61,8 +64,9 @@ def build(rows, cols, depth, seed, cfg):
     return tn, data, gates
 
 
+@pytest.mark.parametrize("carry", [None, 0, 8], ids=["carry", "no-carry", "carry-tiny"])
 @pytest.mark.parametrize("nslice", [0, 3, 5])
-def test_lowered_plan_reproduces_amplitudes(nslice):
+def test_lowered_plan_reproduces_amplitudes(nslice, carry):
     rows, cols, depth, seed = 3, 3, 5, 4
     cfg = "10x1x0110"
     tn, data, _ = build(rows, cols, depth, seed, cfg)
@@ -70,7 +74,7 @@ def test_lowered_plan_reproduces_amplitudes(nslice):
     open_q = [i + 1 for i, ch in enumerate(cfg) if ch == "x"]
     plan = PL.sweep_plan(tn, PL.grid_site_order(rows, cols, open_last=open_q))
     sliced = PL.choose_slices(tn, plan, nslice)[0] if nslice else []
-    res, out = interpret(tn, data, plan, sliced)
+    res, out = interpret(tn, data, plan, sliced, carry)
     assert len(out.idx) == 2
     for j in range(4):
         bits = [int(ch) if ch != "x" else 0 for ch in cfg]
@@ -106,7 +110,8 @@ def test_slicer_cost_matches_recount():
     bonds, tot, peak = PL.choose_slices(tn, plan, 4)
     assert len(set(bonds)) == 4
     # recount from the lowered plan: dependent steps run once per slice
-    slots, steps, _ = lower_plan(tn, plan, bonds)
+    # (the slicer's model batches every step that touches a sliced bond)
+    slots, steps, _ = lower_plan(tn, plan, bonds, carry_elems=0)
     recount = 0
     for sa, sb, sc, M, N, K in steps:
         m = math.prod(x[0] for x in M) * math.prod(x[0] for x in N) * math.prod(x[0] for x in K)
@@ -114,3 +119,30 @@ def test_slicer_cost_matches_recount():
     assert recount == tot
     unsliced = PL.plan_ledger(tn, plan)["macs"]
     assert tot >= unsliced
+
+
+def test_carried_bonds():
+    """Small world-line intermediates keep their sliced bonds (run once);
+    only large steps are batched over the slices; results agree with and
+    without carrying on a greedy plan too."""
+    tn, data, _ = build(3, 3, 6, 2, "0x" * 4 + "1")
+    plan = PL.sweep_plan(tn, PL.grid_site_order(3, 3))
+    sliced = PL.choose_slices(tn, plan, 4)[0]
+    slots, steps, out_slot = lower_plan(tn, plan, sliced)
+    carried = [s for s in slots if s.carry]
+    assert carried
+    for s in carried:
+        assert not s.dep and sum(1 for v in s.slice_stride if v) == len(s.carry)
+    assert slots[out_slot].dep and not slots[out_slot].carry
+    n_dep = sum(slots[sc].dep for _, _, sc, *_ in steps)
+    n_dep0 = sum(x[0] for x in [(lower_plan(tn, plan, sliced, carry_elems=0)[0][sc].dep,)
+                                for _, _, sc, *_ in steps])
+    assert n_dep < n_dep0
+    a, _ = interpret(tn, data, plan, sliced)
+    b, _ = interpret(tn, data, plan, sliced, 0)
+    np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-15)
+    g, _, _ = PL.bgreedy_single(tn, 1.0, 0.0)
+    sl = PL.choose_slices(tn, g, 3)[0]
+    a, _ = interpret(tn, data, g, sl)
+    b, _ = interpret(tn, data, g, sl, 0)
+    np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-15)
