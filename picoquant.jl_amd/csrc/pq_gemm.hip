@@ -209,6 +209,11 @@ __global__ __launch_bounds__(256) void k_gemm(const C* __restrict__ A, const C* 
 // column (B) of a K tile is one contiguous run, read with 16-byte vector
 // loads.  The next tile is prefetched into registers while the matrix
 // cores work on the current one (register double buffering).
+//
+// LDS holds the tiles as interleaved complex values, row-major [row][k]
+// with an odd pitch of BKT + 1 elements: a lane's (re, im) operand pair is
+// one 8-byte (16-byte for complex128) read, and the 16 lanes of an MFMA
+// operand row fall on distinct banks.
 template <typename C, int BM, int BN, int WGM, int BKT>
 __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C* __restrict__ B,
                                                C* __restrict__ Out, GemmArgs g) {
@@ -224,9 +229,10 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
   constexpr int ACH = (BM * CPR + 255) / 256;     // chunks per lane, A tile
   constexpr int BCH = (BN * CPR + 255) / 256;     // chunks per lane, B tile
   constexpr int KT_MAX = 2048 / BKT;              // K tiles (K <= 2048)
+  constexpr int LDK = BKT + 1;                    // LDS row pitch (complex elements)
 
-  __shared__ R As_r[BKT][BM + 4], As_i[BKT][BM + 4];
-  __shared__ R Bs_r[BKT][BN + 4], Bs_i[BKT][BN + 4];
+  __shared__ C As[BM * LDK];
+  __shared__ C Bs[BN * LDK];
   __shared__ int32_t offAm[BM], offCm[BM], offBn[BN], offCn[BN];
   __shared__ int32_t offAk[KT_MAX], offBk[KT_MAX];
 
@@ -261,6 +267,27 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
   }
   __syncthreads();
 
+  // This lane's chunks of each tile: global row offset (fixed over K) and
+  // LDS position.
+  int32_t ga[ACH], gb[BCH], la[ACH], lb[BCH];
+  bool va[ACH], vb[BCH];
+#pragma unroll
+  for (int q = 0; q < ACH; ++q) {
+    const int id = tid + 256 * q;
+    const int row = id / CPR, cc = id % CPR;
+    va[q] = row < BM && m0 + row < Mt;
+    ga[q] = va[q] ? offAm[row] + cc * VEC : 0;
+    la[q] = row < BM ? row * LDK + cc * VEC : -1;
+  }
+#pragma unroll
+  for (int q = 0; q < BCH; ++q) {
+    const int id = tid + 256 * q;
+    const int col = id / CPR, cc = id % CPR;
+    vb[q] = col < BN && n0 + col < Nt;
+    gb[q] = vb[q] ? offBn[col] + cc * VEC : 0;
+    lb[q] = col < BN ? col * LDK + cc * VEC : -1;
+  }
+
   const int wave = tid / 64, lane = tid % 64;
   const int wm = (wave / WGN) * WM, wn = (wave % WGN) * WN;
   const int li = lane % 16, lk = lane / 16;
@@ -272,90 +299,73 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
   typedef typename std::conditional<sizeof(C) == 8, float4, double2>::type V16;
   V16 ra[ACH], rb[BCH];
   auto load_tile = [&](uint32_t kt) {
+    const int32_t ka = offAk[kt], kb = offBk[kt];
 #pragma unroll
     for (int q = 0; q < ACH; ++q) {
-      const int id = tid + 256 * q;
-      const int row = id / CPR, cc = id % CPR;
       V16 v = {};
-      if (row < BM && m0 + row < Mt)
-        v = *reinterpret_cast<const V16*>(A + offAm[row] + offAk[kt] + cc * VEC);
+      if (va[q]) v = *reinterpret_cast<const V16*>(A + ga[q] + ka);
       ra[q] = v;
     }
 #pragma unroll
     for (int q = 0; q < BCH; ++q) {
-      const int id = tid + 256 * q;
-      const int col = id / CPR, cc = id % CPR;
       V16 v = {};
-      if (col < BN && n0 + col < Nt)
-        v = *reinterpret_cast<const V16*>(B + offBn[col] + offBk[kt] + cc * VEC);
+      if (vb[q]) v = *reinterpret_cast<const V16*>(B + gb[q] + kb);
       rb[q] = v;
+    }
+  };
+  auto put = [](C* dst, const V16& v) {
+    if constexpr (VEC == 2) {
+      C x, y;
+      x.x = v.x;
+      x.y = v.y;
+      y.x = v.z;
+      y.y = v.w;
+      dst[0] = x;
+      dst[1] = y;
+    } else {
+      C x;
+      x.x = v.x;
+      x.y = v.y;
+      dst[0] = x;
     }
   };
   auto store_tile = [&]() {
 #pragma unroll
-    for (int q = 0; q < ACH; ++q) {
-      const int id = tid + 256 * q;
-      const int row = id / CPR, k = (id % CPR) * VEC;
-      if (row < BM) {
-        if constexpr (VEC == 2) {
-          As_r[k][row] = ra[q].x;
-          As_i[k][row] = ra[q].y;
-          As_r[k + 1][row] = ra[q].z;
-          As_i[k + 1][row] = ra[q].w;
-        } else {
-          As_r[k][row] = ra[q].x;
-          As_i[k][row] = ra[q].y;
-        }
-      }
-    }
+    for (int q = 0; q < ACH; ++q)
+      if (la[q] >= 0) put(As + la[q], ra[q]);
 #pragma unroll
-    for (int q = 0; q < BCH; ++q) {
-      const int id = tid + 256 * q;
-      const int col = id / CPR, k = (id % CPR) * VEC;
-      if (col < BN) {
-        if constexpr (VEC == 2) {
-          Bs_r[k][col] = rb[q].x;
-          Bs_i[k][col] = rb[q].y;
-          Bs_r[k + 1][col] = rb[q].z;
-          Bs_i[k + 1][col] = rb[q].w;
-        } else {
-          Bs_r[k][col] = rb[q].x;
-          Bs_i[k][col] = rb[q].y;
-        }
-      }
-    }
+    for (int q = 0; q < BCH; ++q)
+      if (lb[q] >= 0) put(Bs + lb[q], rb[q]);
   };
 
+  const C* Ap = As + (wm + li) * LDK + lk;
+  const C* Bp = Bs + (wn + li) * LDK + lk;
+  const typename MF::acc zero = {};
   load_tile(0);
   for (uint32_t kt = 0; kt < ntiles; ++kt) {
     typename MF::acc cr[TM][TN], ci[TM][TN];
-    if constexpr (CHUNKED) zero_acc<MF, TM, TN>(cr, ci);
     store_tile();
     __syncthreads();
     if (kt + 1 < ntiles) load_tile(kt + 1);
 #pragma unroll
     for (int kk = 0; kk < BKT; kk += 4) {
-      R ar[TM], ai[TM], br[TN], bi[TN];
+      C a[TM], b[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        ar[i] = As_r[kk + lk][wm + 16 * i + li];
-        ai[i] = As_i[kk + lk][wm + 16 * i + li];
-      }
+      for (int i = 0; i < TM; ++i) a[i] = Ap[16 * i * LDK + kk];
 #pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        br[j] = Bs_r[kk + lk][wn + 16 * j + li];
-        bi[j] = Bs_i[kk + lk][wn + 16 * j + li];
-      }
+      for (int j = 0; j < TN; ++j) b[j] = Bp[16 * j * LDK + kk];
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
           typename MF::acc& xr = CHUNKED ? cr[i][j] : accr[i][j];
           typename MF::acc& xi = CHUNKED ? ci[i][j] : acci[i][j];
-          xr = MF::op(ar[i], br[j], xr);
-          xr = MF::op(-ai[i], bi[j], xr);
-          xi = MF::op(ar[i], bi[j], xi);
-          xi = MF::op(ai[i], br[j], xi);
+          // a K tile's first products start the tile's fresh sums
+          const bool fresh = CHUNKED && kk == 0;
+          xr = MF::op(a[i].x, b[j].x, fresh ? zero : xr);
+          xr = MF::op(-a[i].y, b[j].y, xr);
+          xi = MF::op(a[i].x, b[j].y, fresh ? zero : xi);
+          xi = MF::op(a[i].y, b[j].x, xi);
         }
     }
     if constexpr (CHUNKED) add_chunk<MF, TM, TN>(accr, acci, cr, ci);
@@ -382,7 +392,7 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
 // K-tile depth of the fast path: 32 for complex64 tiles whose LDS allows it.
 template <typename C, int BM, int BN>
 constexpr int fast_bk() {
-  return (sizeof(C) == 8 && BM + BN <= 192) ? 32 : 16;
+  return (sizeof(C) == 8 && (BM + BN) * 33 * 8 + 16 * (BM + BN) <= 60 * 1024) ? 32 : 16;
 }
 
 // Static LDS stays within 64 KiB: complex128 tiles up to 128 rows or
