@@ -205,6 +205,21 @@ int pq_mlp_loss_grad(pq_ctx* ctx, pq_mlp* mlp, const float* X, const float* y, i
 int pq_mlp_train(pq_ctx* ctx, pq_mlp* mlp, const float* X, const float* y, int n, int epochs, int batch,
                  float lr, uint64_t seed, float* final_loss);
 
+/* ------------------------------------------------------------------------
+ * Optimal contraction order for small networks: replaces the
+ * TensorOperations.optimaltree call behind netcon
+ * (src/layer2/netcon_contraction.jl:14-41).  Host only (no context, no GPU).
+ * Tensor t has rank[t] indices; labels / dims hold them all, tensor after
+ * tensor.  A label in two tensors is summed, a label in one stays open.
+ * Writes the cheapest pair plan by total complex multiply-adds
+ * (prod of dims of A's and B's indices per step): plan[2 s], plan[2 s + 1]
+ * name the operands of step s, ids < n_tensors are inputs and id
+ * n_tensors + s is the result of step s; 2 (n_tensors - 1) entries.
+ * PQ_ERR_UNSUPPORTED when the search exceeds max_work pair evaluations
+ * (<= 0: 2^32) or n_tensors > 64. */
+int pq_netcon(int n_tensors, const int32_t* rank, const int64_t* labels, const int64_t* dims, int64_t max_work,
+              int32_t* plan, double* cost);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,7 @@ Pins (what checks each part; see tests/test_oracle.py):
   * slicing .................................. sum over slices == unsliced
   * decompose_gate ............................ round trip (test/layer3_tests.jl:57-71)
   * bgreedy tau->0 ............................ plan validity + ledger re-count
+  * optimal_contraction_cost .................. hand-counted costs of tiny chains / stars
   * gate tensors .............................. closed forms + unitarity
 """
 
@@ -596,6 +597,56 @@ def greedy_plan(tn, alpha=1.0):
         time_cost += costs[0]
         space_cost = max(space_cost, costs[1])
     return plan, time_cost, space_cost
+
+
+# --------------------------------------------------------------------------- #
+# Optimal contraction cost, exhaustive (the quantity netcon minimises:
+# src/layer2/netcon_contraction.jl:14-41 via TensorOperations.optimaltree)
+# --------------------------------------------------------------------------- #
+
+
+def optimal_contraction_cost(tensors, outer=True):
+    """Minimum over ALL binary contraction trees of the total multiply-adds,
+    each step costing prod(dims of A's and B's indices, shared ones once) --
+    Metrics.flops (src/backends.jl:37-38).  outer=False admits only steps
+    whose operands share an index (for connected networks).  tensors:
+    [(indices, dims)].  Dynamic programming over every subset and every
+    split; exponential, for a handful of tensors only."""
+    n = len(tensors)
+    dim = {}
+    count = {}
+    for idx, dims in tensors:
+        for i, d in zip(idx, dims):
+            dim[i] = d
+            count[i] = count.get(i, 0) + 1
+    members = {}
+    for t, (idx, _) in enumerate(tensors):
+        for i in idx:
+            members.setdefault(i, set()).add(t)
+
+    def open_set(mask):
+        ts = {t for t in range(n) if mask >> t & 1}
+        return {i for t in ts for i in tensors[t][0] if not members[i] <= ts or count[i] == 1}
+
+    opened = {m: open_set(m) for m in range(1, 1 << n)}
+    best = {1 << t: 0 for t in range(n)}
+    for m in range(1, 1 << n):   # a subset's splits are smaller subsets: ascending order works
+        if m & (m - 1) == 0:
+            continue
+        low = m & -m
+        b = None
+        sub = (m - 1) & m
+        while sub:
+            rest = m ^ sub
+            if sub & low and best[sub] is not None and best[rest] is not None and (
+                    outer or opened[sub] & opened[rest]):     # each split once
+                macs = math.prod(dim[i] for i in opened[sub] | opened[rest])
+                c = best[sub] + best[rest] + macs
+                if b is None or c < b:
+                    b = c
+            sub = (sub - 1) & m
+        best[m] = b
+    return best[(1 << n) - 1]
 
 
 # --------------------------------------------------------------------------- #

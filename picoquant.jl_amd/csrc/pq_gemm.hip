@@ -231,8 +231,8 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
   constexpr int KT_MAX = 2048 / BKT;              // K tiles (K <= 2048)
   constexpr int LDK = BKT + 1;                    // LDS row pitch (complex elements)
 
-  __shared__ C As[BM * LDK];
-  __shared__ C Bs[BN * LDK];
+  __shared__ C As[BM * LDK + VEC];   // + a spare chunk written by idle lanes
+  __shared__ C Bs[BN * LDK + VEC];
   __shared__ int32_t offAm[BM], offCm[BM], offBn[BN], offCn[BN];
   __shared__ int32_t offAk[KT_MAX], offBk[KT_MAX];
 
@@ -267,25 +267,25 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
   }
   __syncthreads();
 
-  // This lane's chunks of each tile: global row offset (fixed over K) and
-  // LDS position.
+  // This lane's chunks of each tile: global offset (fixed over K) and LDS
+  // position.  Loads and stores are unconditional: rows past the operand's
+  // extent read its first row (offAm / offBn hold 0 there), and their
+  // products only reach outputs that are never stored; lanes past the tile
+  // (when 256 does not divide its chunk count) write a spare LDS chunk.
   int32_t ga[ACH], gb[BCH], la[ACH], lb[BCH];
-  bool va[ACH], vb[BCH];
 #pragma unroll
   for (int q = 0; q < ACH; ++q) {
     const int id = tid + 256 * q;
     const int row = id / CPR, cc = id % CPR;
-    va[q] = row < BM && m0 + row < Mt;
-    ga[q] = va[q] ? offAm[row] + cc * VEC : 0;
-    la[q] = row < BM ? row * LDK + cc * VEC : -1;
+    ga[q] = (row < BM ? offAm[row] : 0) + cc * VEC;
+    la[q] = row < BM ? row * LDK + cc * VEC : BM * LDK;
   }
 #pragma unroll
   for (int q = 0; q < BCH; ++q) {
     const int id = tid + 256 * q;
     const int col = id / CPR, cc = id % CPR;
-    vb[q] = col < BN && n0 + col < Nt;
-    gb[q] = vb[q] ? offBn[col] + cc * VEC : 0;
-    lb[q] = col < BN ? col * LDK + cc * VEC : -1;
+    gb[q] = (col < BN ? offBn[col] : 0) + cc * VEC;
+    lb[q] = col < BN ? col * LDK + cc * VEC : BN * LDK;
   }
 
   const int wave = tid / 64, lane = tid % 64;
@@ -301,17 +301,9 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
   auto load_tile = [&](uint32_t kt) {
     const int32_t ka = offAk[kt], kb = offBk[kt];
 #pragma unroll
-    for (int q = 0; q < ACH; ++q) {
-      V16 v = {};
-      if (va[q]) v = *reinterpret_cast<const V16*>(A + ga[q] + ka);
-      ra[q] = v;
-    }
+    for (int q = 0; q < ACH; ++q) ra[q] = *reinterpret_cast<const V16*>(A + ga[q] + ka);
 #pragma unroll
-    for (int q = 0; q < BCH; ++q) {
-      V16 v = {};
-      if (vb[q]) v = *reinterpret_cast<const V16*>(B + gb[q] + kb);
-      rb[q] = v;
-    }
+    for (int q = 0; q < BCH; ++q) rb[q] = *reinterpret_cast<const V16*>(B + gb[q] + kb);
   };
   auto put = [](C* dst, const V16& v) {
     if constexpr (VEC == 2) {
@@ -331,11 +323,9 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
   };
   auto store_tile = [&]() {
 #pragma unroll
-    for (int q = 0; q < ACH; ++q)
-      if (la[q] >= 0) put(As + la[q], ra[q]);
+    for (int q = 0; q < ACH; ++q) put(As + la[q], ra[q]);
 #pragma unroll
-    for (int q = 0; q < BCH; ++q)
-      if (lb[q] >= 0) put(Bs + lb[q], rb[q]);
+    for (int q = 0; q < BCH; ++q) put(Bs + lb[q], rb[q]);
   };
 
   const C* Ap = As + (wm + li) * LDK + lk;

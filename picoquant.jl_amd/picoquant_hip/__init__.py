@@ -13,6 +13,7 @@ from .contraction import (contract_network, contract_pair, create_ncon_indices,
                           full_wavefunction_contraction, random_contraction_plan, sort_indices)
 from .slicing import (multi_index_partition, partition_network_on_virtual_bonds,
                       replace_with_view, slice_tensor_network)
+from .optimal_order import netcon, netcon_contraction
 from . import circuits
 
 __all__ = [
@@ -22,4 +23,5 @@ __all__ = [
     "contract_network", "contract_pair", "create_ncon_indices", "full_wavefunction_contraction",
     "random_contraction_plan", "sort_indices", "multi_index_partition",
     "partition_network_on_virtual_bonds", "replace_with_view", "slice_tensor_network", "circuits",
+    "netcon", "netcon_contraction",
 ]

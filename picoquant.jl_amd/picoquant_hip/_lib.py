@@ -75,6 +75,8 @@ SIGNATURES = {
                                     ct.POINTER(ct.c_float), ct.POINTER(ct.c_float)]),
     "pq_mlp_train": (ct.c_int, [_P, _P, ct.POINTER(ct.c_float), ct.POINTER(ct.c_float), ct.c_int, ct.c_int,
                                 ct.c_int, ct.c_float, ct.c_uint64, ct.POINTER(ct.c_float)]),
+    "pq_netcon": (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64),
+                             ct.c_int64, ct.POINTER(ct.c_int32), ct.POINTER(ct.c_double)]),
 }
 
 _lib = None

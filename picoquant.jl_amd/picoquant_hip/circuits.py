@@ -32,6 +32,8 @@ SQRT_Y = sqrt_involution(Y)
 SQRT_W = sqrt_involution(W)
 SQRT_GATES = (SQRT_X, SQRT_Y, SQRT_W)
 CZ = np.diag([1, 1, 1, -1]).astype(np.complex128)
+# CX on [control, target]: control is the first (least significant) qubit
+CX = np.array([[1, 0, 0, 0], [0, 0, 0, 1], [0, 0, 1, 0], [0, 1, 0, 0]], dtype=np.complex128)
 SWAP = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], dtype=np.complex128)
 
 

@@ -175,3 +175,28 @@ def test_backend_verbs(dtype):
     be.delete_tensor("g")
     with pytest.raises(KeyError):
         be.contract_tensors("nope", [1], "l", [1, -1, -2], "c")
+
+
+def test_netcon_contraction(dtype):
+    """netcon_contraction! on the reference's netcon test network (H, CX, CX;
+    input and output 000, test/layer2_tests.jl:145-170): one node is left and
+    <000|GHZ> = 1/sqrt(2); a 3x3 grid amplitude through netcon's plan matches
+    the brute-force state vector."""
+    from picoquant_hip import optimal_order as NC
+    tn = P.network_from_gates(3, [(C.H, [0]), (C.CX, [0, 1]), (C.CX, [0, 2])], backend=P.HIPBackend(dtype))
+    P.add_input(tn, "000")
+    P.add_output(tn, "000")
+    NC.netcon_contraction(tn)
+    assert len(tn.nodes) == 1
+    amp = complex(np.asarray(tn.backend.load_tensor_data("result")).reshape(-1)[0])
+    assert abs(amp - 1 / math.sqrt(2)) < RTOL[dtype]
+    bits = "101100110"
+    tn = P.network_from_gates(9, C.grid_circuit_gates(3, 3, 1, 7), backend=P.HIPBackend(dtype), decompose=True)
+    P.add_input(tn, "0" * 9)
+    P.add_output(tn, bits)
+    assert len(tn.nodes) <= NC.MAX_NODES
+    NC.netcon_contraction(tn)
+    amp = complex(np.asarray(tn.backend.load_tensor_data("result")).reshape(-1)[0])
+    sv = O.statevector(9, O.grid_circuit_gates(3, 3, 1, 7))
+    want = sv[sum(int(b) << q for q, b in enumerate(bits))]
+    assert rel_err([amp], [want]) < RTOL[dtype]
