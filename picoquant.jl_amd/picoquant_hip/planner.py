@@ -391,16 +391,9 @@ def _sliced_costs(mats, chosen, nslices, model):
     return cost, peak
 
 
-def choose_slices(network, plan, n_bonds, max_elems=2 ** 28, exclude=(), candidates=200, model="time"):
-    """Greedy choice of `n_bonds` extent-2 bonds to slice: each round adds the
-    bond that minimises the total sliced cost (steps that do not depend on a
-    sliced bond run once, the others once per slice), preferring choices that
-    meet the per-slice size cap.  model: "time" (roofline seconds) or "macs".
-    Returns (bonds, total_macs, peak)."""
-    steps, ids, ext = _plan_masks(network, plan)
-    if n_bonds == 0:
-        tot, peak = sliced_cost(steps, ext, 0, 1, "macs")
-        return [], tot, peak
+def _bond_pool(network, steps, ids, ext, exclude, candidates):
+    """Extent-2 bonds worth slicing: those of the largest steps first (open
+    outputs and `exclude` left out), at most `candidates` of them."""
     inv = {v: k for k, v in ids.items()}
     open_out = set()
     for k, e in network.edges.items():
@@ -424,6 +417,20 @@ def choose_slices(network, plan, n_bonds, max_elems=2 ** 28, exclude=(), candida
                 cand.append(j)
         if len(cand) >= candidates:
             break
+    return cand, inv
+
+
+def choose_slices(network, plan, n_bonds, max_elems=2 ** 28, exclude=(), candidates=200, model="time"):
+    """Greedy choice of `n_bonds` extent-2 bonds to slice: each round adds the
+    bond that minimises the total sliced cost (steps that do not depend on a
+    sliced bond run once, the others once per slice), preferring choices that
+    meet the per-slice size cap.  model: "time" (roofline seconds) or "macs".
+    Returns (bonds, total_macs, peak)."""
+    steps, ids, ext = _plan_masks(network, plan)
+    if n_bonds == 0:
+        tot, peak = sliced_cost(steps, ext, 0, 1, "macs")
+        return [], tot, peak
+    cand, inv = _bond_pool(network, steps, ids, ext, exclude, candidates)
     if len(cand) < n_bonds:
         raise ValueError("not enough bonds to slice")
     mats = _bond_matrices(steps, ext, cand)
@@ -442,3 +449,47 @@ def choose_slices(network, plan, n_bonds, max_elems=2 ** 28, exclude=(), candida
         smask |= 1 << cand[q]
     tot, peak = sliced_cost(steps, ext, smask, 2 ** n_bonds, "macs")
     return [inv[cand[q]] for q in chosen], tot, peak
+
+
+def slicing_candidates(network, plan, n_bonds, max_elems=2 ** 28, n_random=24, seed=0, pool=40):
+    """[(bonds, sliced_macs, peak)]: the greedy slicings under the roofline
+    and MAC objectives, plus random exchanges of one or two bonds of the
+    roofline-greedy set for bonds of the same pool, keeping only sets whose
+    per-slice peak is within max_elems (no duplicates)."""
+    out, seen = [], set()
+
+    def add(bonds, tot, peak):
+        key = frozenset(bonds)
+        if key not in seen and peak <= max_elems:
+            seen.add(key)
+            out.append((list(bonds), tot, peak))
+
+    base = None
+    for model in ("time", "macs"):
+        try:
+            b, tot, peak = choose_slices(network, plan, n_bonds, max_elems, model=model)
+        except ValueError:
+            continue
+        base = base or b
+        add(b, tot, peak)
+    if base is None or n_bonds == 0:
+        return out
+    steps, ids, ext = _plan_masks(network, plan)
+    cand, inv = _bond_pool(network, steps, ids, ext, (), pool)
+    rng = np.random.default_rng(seed)
+    names = [inv[j] for j in cand]
+    tries = 0
+    while len(out) < n_random + 2 and tries < 20 * n_random:
+        tries += 1
+        b = list(base)
+        for _ in range(1 + int(rng.integers(2))):
+            free = [x for x in names if x not in b]
+            if not free:
+                break
+            b[int(rng.integers(len(b)))] = free[int(rng.integers(len(free)))]
+        smask = 0
+        for x in b:
+            smask |= 1 << ids[x]
+        tot, peak = sliced_cost(steps, ext, smask, 2 ** n_bonds, "macs")
+        add(b, tot, peak)
+    return out

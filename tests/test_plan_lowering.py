@@ -146,3 +146,18 @@ def test_carried_bonds():
     a, _ = interpret(tn, data, g, sl)
     b, _ = interpret(tn, data, g, sl, 0)
     np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-15)
+
+
+def test_slicing_candidates():
+    tn, _, _ = build(3, 3, 6, 1, "0" * 9)
+    plan = PL.sweep_plan(tn, PL.grid_site_order(3, 3))
+    greedy = PL.choose_slices(tn, plan, 3)[0]
+    cands = PL.slicing_candidates(tn, plan, 3, max_elems=2 ** 10, n_random=10, seed=1)
+    assert sorted(cands[0][0]) == sorted(greedy)
+    keys = [frozenset(b) for b, _, _ in cands]
+    assert len(set(keys)) == len(keys)
+    steps, ids, ext = PL._plan_masks(tn, plan)
+    for bonds, tot, peak in cands:
+        assert len(set(bonds)) == 3 and peak <= 2 ** 10
+        smask = sum(1 << ids[b] for b in bonds)
+        assert (tot, peak) == PL.sliced_cost(steps, ext, smask, 8, "macs")
