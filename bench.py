@@ -88,6 +88,59 @@ def train_cost_model(ctx, n_samples=20000):
     return final, stats
 
 
+def run_variant(tn, plan, sliced, precision, group_bits, runs=3, tune=True):
+    """Autotune (for this precision), compile and time one plan variant:
+    (ms per run, result vector)."""
+    from picoquant_hip import autotune
+    from picoquant_hip.executor import compile_plan
+    ctx = tn.backend.ctx
+    kc = tiles = None
+    if tune and precision != "uniform":
+        kc, tiles, _ = autotune.autotune_plan(ctx, tn, plan, sliced, group_bits=group_bits, precision=precision)
+    p = compile_plan(tn, plan, sliced, group_bits=group_bits, kc=kc, tiles=tiles, precision=precision)
+    t = time_plan(p, ctx, runs)
+    res = p.result().reshape(-1)
+    classes = {}
+    ms = p.profile()
+    from picoquant_hip import _lib
+    for i in range(len(p.steps)):
+        name = _lib.KC_NAMES[p.step_info(i)["kc"]]
+        classes[name] = round(classes.get(name, 0.0) + ms[i], 3)
+    p.close()
+    return 1e3 * t, res, classes
+
+
+def grid7x7_config():
+    """The 49-qubit config: 7x7 grid, 12 cycles (seed 2027), one amplitude
+    of a seeded bitstring, sweep plan with 6 sliced bonds (64 slices,
+    largest per-slice intermediate far below 2^30); FP64, complex64 on FP32
+    MFMA and FP16 hi/lo split, errors against the FP64 run."""
+    import math
+    import picoquant_hip as P
+    from picoquant_hip import circuits as C, planner as PL
+    r = c = 7
+    n = r * c
+    seed = 2027
+    tn = P.network_from_gates(n, C.grid_circuit_gates(r, c, 12, seed), backend=P.HIPBackend(np.complex128),
+                              decompose=True)
+    P.add_input(tn, "0" * n)
+    P.add_output(tn, "".join(map(str, C.seeded_bits(n, seed + 1))))
+    plan = PL.sweep_plan(tn, PL.grid_site_order(r, c, "col"))
+    sliced, macs, peak = PL.choose_slices(tn, plan, 6, max_elems=2 ** 30)
+    out = {"workload": "7x7 grid depth-12 random circuit (seed 2027), single amplitude, 6 sliced bonds = 64 slices",
+           "sliced_macs_log2": round(math.log2(macs), 2), "peak_slice_elems_log2": round(math.log2(peak), 1)}
+    ref = None
+    for prec in ("uniform", "mixed", "mixed16"):
+        ms, res, cls = run_variant(tn, plan, sliced, prec, 6)
+        if ref is None:
+            ref = res
+        err = float(np.max(np.abs(res - ref) / np.abs(ref)))
+        out[{"uniform": "fp64", "mixed": "fp32_mfma", "mixed16": "fp16_split"}[prec]] = {
+            "ms": round(ms, 3), "rel_err_vs_fp64": err, "classes_ms": cls}
+    out["amplitude"] = [float(ref[0].real), float(ref[0].imag)]
+    return out
+
+
 def choose_plan(tn, open_q, model=None):
     from picoquant_hip import learned
     return learned.select_plan(tn, ROWS, COLS, open_q, N_SLICED, model=model)
@@ -207,6 +260,9 @@ def main():
     ap.add_argument("--tuned-cache", default="", metavar="JSON",
                     help="reuse the autotuner's per-step choices from this file (written when missing), "
                          "so profiler runs trace only the plan's own kernels")
+    ap.add_argument("--no-grid7", action="store_true", help="skip the 7x7 depth-12 precision comparison")
+    ap.add_argument("--no-split16", action="store_true",
+                    help="skip timing the FP16 hi/lo split variant of the benchmark plan")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the strided-copy kernel over a MIB-MiB complex64 tensor 10 times "
                          "(scale factor for the HBM byte counters)")
@@ -324,6 +380,20 @@ def main():
     p64.close()
     rel_err = float(np.max(np.abs(res64 - ref) / np.abs(ref)))
     rel_err_norm = float(np.linalg.norm(res64 - ref) / np.linalg.norm(ref))
+
+    # The same plan with the complex64 GEMMs on FP16 hi/lo matrix cores.
+    split16 = None
+    if not args.no_split16:
+        ms16, res16, cls16 = run_variant(tn, plan, sliced, "mixed16", N_SLICED)
+        split16 = {"ms_per_run": round(ms16, 3), "amps_per_s": 64e3 / ms16,
+                   "rel_err_vs_fp64": float(np.max(np.abs(res16 - ref) / np.abs(ref))),
+                   "rel_err_vs_fp64_normwise": float(np.linalg.norm(res16 - ref) / np.linalg.norm(ref)),
+                   "classes_ms": cls16}
+        log(f"[bench] FP16 hi/lo split variant: {split16}")
+    grid7 = None
+    if not args.no_grid7:
+        grid7 = grid7x7_config()
+        log(f"[bench] 7x7 d12: {grid7}")
     log(f"[bench] max rel err vs FP64 {rel_err:.2e} (norm-wise {rel_err_norm:.2e}); "
         f"batched FP64 run {t_fp64_batched:.3f}s")
 
@@ -364,6 +434,8 @@ def main():
         "speedup_vs_plain": (amps / plain["amps_per_s"]) if plain else None,
         "batched_fp64_amps_per_s": 64 / t_fp64_batched,
         "planner": planner_info,
+        "fp16_split_variant": split16,
+        "grid7x7_d12": grid7,
     }
     print(json.dumps(out))
 

@@ -100,6 +100,7 @@ int pq_tensor_device_ptr(pq_ctx* ctx, pq_tensor t, void** base, int64_t* elem_of
 #define PQ_KC_COPY 4     /* strided copy / permutation */
 #define PQ_KC_BATCHED 5  /* batched small contractions (one group per problem) */
 #define PQ_KC_ACCUM 6    /* FP64 cross-slice accumulation */
+#define PQ_KC_GEMM16 7   /* complex64 GEMM on FP16 hi/lo matrix cores, FP32 accumulation */
 #define PQ_KC_COUNT 8
 int pq_profile_enable(pq_ctx* ctx, int enable);
 int pq_profile_read(pq_ctx* ctx, int kclass, int64_t* launches, double* total_ms,
@@ -116,6 +117,12 @@ int pq_set_kernel_override(pq_ctx* ctx, int kclass);
  * (complex128 tiles are capped at 4096 elements and 128 rows or columns). */
 #define PQ_GEMM_TILES 15
 int pq_set_gemm_tile(pq_ctx* ctx, int tile);
+
+/* Arithmetic of complex64 GEMM steps for pq_contract / pq_time_step:
+ * 0 = FP32 matrix cores (default), 1 = FP16 hi/lo split with FP32
+ * accumulation where the shape allows it (K contiguous in both operands, a
+ * multiple of 32; other steps keep FP32).  Plans set it per step. */
+int pq_set_arithmetic(pq_ctx* ctx, int mode);
 
 /* GEMM orientation override: -1 = automatic (the product is launched
  * transposed, C^T = B^T A^T, when C's unit stride lies on A's open side, so
@@ -160,6 +167,8 @@ typedef struct {
                                        leaves must have the context's type */
   const int32_t* step_kind;         /* [n_steps] 0 = contract A x B -> C, 1 = convert A -> C
                                        (element type change, same layout; NULL = all contract) */
+  const int32_t* step_arith;        /* [n_steps] complex64 GEMM arithmetic as for pq_set_arithmetic
+                                       (NULL = 0, FP32 matrix cores) */
 } pq_plan_desc;
 
 int pq_plan_create(pq_ctx* ctx, const pq_plan_desc* desc, pq_plan** out);

@@ -428,6 +428,223 @@ hipError_t launch_tile(const C* A, const C* B, C* Out, const GemmArgs& g, hipStr
   }
 }
 
+// ----------------------------------------------------------------- k_gemm_h
+// complex64 GEMM on FP16 matrix cores with a hi/lo split (the fast path's
+// contiguous-K layout; K tiles of 32).  Per K tile every row of A (column of
+// B) gets a power-of-two scale 2^s that brings its largest magnitude into
+// [2^14, 2^15); each scaled FP32 value x is split into hi = fp16(x) and
+// lo = fp16(x - hi), and the product is summed as hi*hi + hi*lo + lo*hi
+// (lo*lo dropped, about 2^-22 relative) on v_mfma_f32_16x16x32_f16 with FP32
+// accumulation: 12 FP16 MFMAs per 16x16 complex block and K tile.  The
+// tile's sums are unscaled exactly (ldexp) and added to the running FP32
+// sums, as in the FP32 path's two-level accumulation.
+//
+// MFMA 16x16x32 f16 lane layout (64 lanes): A lane l holds A[l%16][8(l/16)+i],
+// B lane l holds B[8(l/16)+i][l%16], accumulator r holds D[4(l/16)+r][l%16].
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ floatx4 mfma16(const half8& a, const half8& b, const floatx4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+template <int BM, int BN, int WGM>
+__global__ __launch_bounds__(256) void k_gemm_h(const c64* __restrict__ A, const c64* __restrict__ B,
+                                                c64* __restrict__ Out, GemmArgs g) {
+  constexpr int BKT = 32;                         // complex K per tile = one MFMA depth
+  constexpr int WGN = 4 / WGM;
+  constexpr int WM = BM / WGM, WN = BN / WGN;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  static_assert(WM % 16 == 0 && WN % 16 == 0, "wave tile must be a multiple of 16");
+  constexpr int CPR = BKT / 2;                    // 16-byte chunks (2 complex) per row
+  constexpr int ACH = (BM * CPR + 255) / 256;
+  constexpr int BCH = (BN * CPR + 255) / 256;
+  constexpr int KT_MAX = 2048 / BKT;
+  constexpr int PK = BKT + 8;                     // plane pitch in halves (16-byte aligned rows)
+
+  // planes: 0 re-hi, 1 re-lo, 2 im-hi, 3 im-lo; one spare row for idle lanes
+  __shared__ __align__(16) _Float16 Ap[4][(BM + 1) * PK];
+  __shared__ __align__(16) _Float16 Bp[4][(BN + 1) * PK];
+  __shared__ __align__(16) int32_t sA[BM + 1], sB[BN + 1];
+  __shared__ int32_t offAm[BM], offCm[BM], offBn[BN], offCn[BN];
+  __shared__ int32_t offAk[KT_MAX], offBk[KT_MAX];
+
+  const uint32_t Mt = g.M.total, Nt = g.N.total, Kt = g.K.total;
+  const uint32_t m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int tid = threadIdx.x;
+  {
+    int64_t oa, ob, oc;
+    batch_offsets(g.L, blockIdx.z, oa, ob, oc);
+    A += oa;
+    B += ob;
+    Out += oc;
+  }
+  for (int t = tid; t < BM; t += 256) {
+    int32_t oa = 0, oc = 0;
+    if (m0 + t < Mt) group_offsets(g.M, m0 + t, oa, oc);
+    offAm[t] = oa;
+    offCm[t] = oc;
+  }
+  for (int t = tid; t < BN; t += 256) {
+    int32_t ob = 0, oc = 0;
+    if (n0 + t < Nt) group_offsets(g.N, n0 + t, ob, oc);
+    offBn[t] = ob;
+    offCn[t] = oc;
+  }
+  const uint32_t ntiles = Kt / BKT;
+  for (uint32_t t = tid; t < ntiles; t += 256) {
+    int32_t oa, ob;
+    group_offsets(g.K, t * BKT, oa, ob);
+    offAk[t] = oa;
+    offBk[t] = ob;
+  }
+  __syncthreads();
+
+  // Lane chunks: 16 consecutive lanes hold the 16 chunks of one row, so a
+  // row's scale is a reduction over a 16-lane group.  Rows past the
+  // operand read its first row; lanes past the tile use the spare row.
+  int32_t ga[ACH], gb[BCH], la[ACH], lb[BCH], ra_row[ACH], rb_row[BCH];
+#pragma unroll
+  for (int q = 0; q < ACH; ++q) {
+    const int id = tid + 256 * q;
+    const int row = id / CPR, cc = id % CPR;
+    ga[q] = (row < BM ? offAm[row] : 0) + cc * 2;
+    ra_row[q] = row < BM ? row : BM;
+    la[q] = ra_row[q] * PK + cc * 2;
+  }
+#pragma unroll
+  for (int q = 0; q < BCH; ++q) {
+    const int id = tid + 256 * q;
+    const int col = id / CPR, cc = id % CPR;
+    gb[q] = (col < BN ? offBn[col] : 0) + cc * 2;
+    rb_row[q] = col < BN ? col : BN;
+    lb[q] = rb_row[q] * PK + cc * 2;
+  }
+  const bool lead = (tid % CPR) == 0;
+
+  const int wave = tid / 64, lane = tid % 64;
+  const int wm = (wave / WGN) * WM, wn = (wave % WGN) * WN;
+  const int li = lane % 16, lk = lane / 16;
+
+  floatx4 accr[TM][TN], acci[TM][TN];
+  zero_acc<mfma_t<float>, TM, TN>(accr, acci);
+
+  float4 ra[ACH], rb[BCH];
+  auto load_tile = [&](uint32_t kt) {
+    const int32_t ka = offAk[kt], kb = offBk[kt];
+#pragma unroll
+    for (int q = 0; q < ACH; ++q) ra[q] = *reinterpret_cast<const float4*>(A + ga[q] + ka);
+#pragma unroll
+    for (int q = 0; q < BCH; ++q) rb[q] = *reinterpret_cast<const float4*>(B + gb[q] + kb);
+  };
+  // Scale, split and store one chunk (2 complex) of a row into the planes.
+  auto split_store = [&](_Float16* p0, _Float16* p1, _Float16* p2, _Float16* p3, int at, const float4& v,
+                         int* srow, int row, bool lead_lane) {
+    float m = fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w)));
+    m = fmaxf(m, __shfl_xor(m, 8, 16));
+    m = fmaxf(m, __shfl_xor(m, 4, 16));
+    m = fmaxf(m, __shfl_xor(m, 2, 16));
+    m = fmaxf(m, __shfl_xor(m, 1, 16));
+    int e = 0;
+    (void)frexpf(m, &e);
+    const int sc = m > 0.f ? 15 - e : 0;         // max * 2^sc in [2^14, 2^15)
+    const float xr0 = ldexpf(v.x, sc), xi0 = ldexpf(v.y, sc), xr1 = ldexpf(v.z, sc), xi1 = ldexpf(v.w, sc);
+    half2v hr, lr, hi, li2;
+    hr.x = (_Float16)xr0;
+    hr.y = (_Float16)xr1;
+    hi.x = (_Float16)xi0;
+    hi.y = (_Float16)xi1;
+    lr.x = (_Float16)(xr0 - (float)hr.x);
+    lr.y = (_Float16)(xr1 - (float)hr.y);
+    li2.x = (_Float16)(xi0 - (float)hi.x);
+    li2.y = (_Float16)(xi1 - (float)hi.y);
+    *reinterpret_cast<half2v*>(p0 + at) = hr;
+    *reinterpret_cast<half2v*>(p1 + at) = lr;
+    *reinterpret_cast<half2v*>(p2 + at) = hi;
+    *reinterpret_cast<half2v*>(p3 + at) = li2;
+    if (lead_lane) srow[row] = sc;
+  };
+
+  load_tile(0);
+  for (uint32_t kt = 0; kt < ntiles; ++kt) {
+#pragma unroll
+    for (int q = 0; q < ACH; ++q) split_store(Ap[0], Ap[1], Ap[2], Ap[3], la[q], ra[q], sA, ra_row[q], lead);
+#pragma unroll
+    for (int q = 0; q < BCH; ++q) split_store(Bp[0], Bp[1], Bp[2], Bp[3], lb[q], rb[q], sB, rb_row[q], lead);
+    __syncthreads();
+    if (kt + 1 < ntiles) load_tile(kt + 1);
+    half8 a[TM][4], an[TM][2], b[TN][4];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int at = (wm + 16 * i + li) * PK + 8 * lk;
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) a[i][pl] = *reinterpret_cast<const half8*>(&Ap[pl][at]);
+      an[i][0] = -a[i][2];
+      an[i][1] = -a[i][3];
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int at = (wn + 16 * j + li) * PK + 8 * lk;
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) b[j][pl] = *reinterpret_cast<const half8*>(&Bp[pl][at]);
+    }
+    const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      // row scales of this lane's four output rows
+      const int4 sa = *reinterpret_cast<const int4*>(&sA[wm + 16 * i + 4 * lk]);
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        floatx4 cr = mfma16(a[i][0], b[j][1], zero);      // re: hi*lo + lo*hi + hi*hi
+        cr = mfma16(a[i][1], b[j][0], cr);
+        cr = mfma16(an[i][0], b[j][3], cr);                // - im * im
+        cr = mfma16(an[i][1], b[j][2], cr);
+        cr = mfma16(a[i][0], b[j][0], cr);
+        cr = mfma16(an[i][0], b[j][2], cr);
+        floatx4 ci = mfma16(a[i][0], b[j][3], zero);       // im: re*im + im*re
+        ci = mfma16(a[i][1], b[j][2], ci);
+        ci = mfma16(a[i][2], b[j][1], ci);
+        ci = mfma16(a[i][3], b[j][0], ci);
+        ci = mfma16(a[i][0], b[j][2], ci);
+        ci = mfma16(a[i][2], b[j][0], ci);
+        const int sb = sB[wn + 16 * j + li];
+        accr[i][j][0] += ldexpf(cr[0], -(sa.x + sb));
+        acci[i][j][0] += ldexpf(ci[0], -(sa.x + sb));
+        accr[i][j][1] += ldexpf(cr[1], -(sa.y + sb));
+        acci[i][j][1] += ldexpf(ci[1], -(sa.y + sb));
+        accr[i][j][2] += ldexpf(cr[2], -(sa.z + sb));
+        acci[i][j][2] += ldexpf(ci[2], -(sa.z + sb));
+        accr[i][j][3] += ldexpf(cr[3], -(sa.w + sb));
+        acci[i][j][3] += ldexpf(ci[3], -(sa.w + sb));
+      }
+    }
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int m = wm + 16 * i + 4 * lk + r;
+        int n = wn + 16 * j + li;
+        if (m0 + m < Mt && n0 + n < Nt) {
+          c64 v;
+          v.x = accr[i][j][r];
+          v.y = acci[i][j][r];
+          Out[offCm[m] + offCn[n]] = v;
+        }
+      }
+}
+
+template <int BM, int BN, int WGM>
+hipError_t launch_tile_h(const c64* A, const c64* B, c64* Out, const GemmArgs& g, hipStream_t s) {
+  dim3 grid((g.M.total + BM - 1) / BM, (g.N.total + BN - 1) / BN, g.L.total);
+  hipLaunchKernelGGL((k_gemm_h<BM, BN, WGM>), grid, dim3(256), 0, s, A, B, Out, g);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 int gemm_k_limit() { return KTAB_MAX; }
@@ -452,6 +669,29 @@ hipError_t launch_gemm(const C* A, const C* B, C* Out, const GemmArgs& g, hipStr
     case 13: return launch_tile<C, 16, 256, 1>(A, B, Out, g, s);
     case 14: return launch_tile<C, 32, 128, 1>(A, B, Out, g, s);
     default: return hipErrorInvalidValue;
+  }
+}
+
+bool gemm16_applicable(const GemmArgs& g) {
+  return g.contig && g.K.nmodes > 0 && g.K.ext[0].d % 32 == 0 && g.K.total <= (uint32_t)KTAB_MAX &&
+         g.L.total >= 1 && g.L.total <= 65535;
+}
+
+hipError_t launch_gemm16(const c64* A, const c64* B, c64* Out, const GemmArgs& g, hipStream_t s) {
+  if (!gemm16_applicable(g)) return hipErrorInvalidValue;
+  switch (g.tile) {
+    case 0: return launch_tile_h<32, 32, 2>(A, B, Out, g, s);
+    case 1: return launch_tile_h<64, 32, 2>(A, B, Out, g, s);
+    case 2: return launch_tile_h<32, 64, 2>(A, B, Out, g, s);
+    case 4: return launch_tile_h<128, 64, 2>(A, B, Out, g, s);
+    case 5: return launch_tile_h<64, 128, 2>(A, B, Out, g, s);
+    case 6: return launch_tile_h<64, 16, 4>(A, B, Out, g, s);
+    case 7: return launch_tile_h<128, 16, 4>(A, B, Out, g, s);
+    case 9: return launch_tile_h<16, 64, 1>(A, B, Out, g, s);
+    case 10: return launch_tile_h<128, 32, 4>(A, B, Out, g, s);
+    case 12: return launch_tile_h<16, 128, 1>(A, B, Out, g, s);
+    case 14: return launch_tile_h<32, 128, 1>(A, B, Out, g, s);
+    default: return launch_tile_h<64, 64, 2>(A, B, Out, g, s);     // 3, and 8 / 11 / 13 (LDS)
   }
 }
 

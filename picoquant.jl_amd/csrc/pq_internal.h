@@ -72,6 +72,9 @@ hipError_t launch_copy(const C* In, C* Out, const CopyArgs& a, hipStream_t s);
 template <typename C>
 hipError_t launch_gemm(const C* A, const C* B, C* Out, const GemmArgs& a, hipStream_t s);
 int gemm_k_limit();
+// complex64 GEMM on FP16 hi/lo matrix cores (contiguous K, multiple of 32).
+bool gemm16_applicable(const GemmArgs& g);
+hipError_t launch_gemm16(const c64* A, const c64* B, c64* Out, const GemmArgs& a, hipStream_t s);
 
 // Number of partial blocks the split-K reduction uses for a given K extent.
 int reduce_blocks_for(uint32_t ktotal);

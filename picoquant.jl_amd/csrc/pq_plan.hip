@@ -369,6 +369,8 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
         L.sC[j] = stride_of(Cs, j);
       }
     }
+    st.spec.arith = d->step_arith ? d->step_arith[i] : 0;
+    if (st.spec.arith < 0 || st.spec.arith > 1) return bad("bad step arithmetic");
     st.spec.kc = d->step_kc ? d->step_kc[i] : -1;
     st.spec.tile = d->step_tile ? d->step_tile[i] : -1;
     int r = resolve_step(c, st.spec, esize_of(st.spec.dtype));

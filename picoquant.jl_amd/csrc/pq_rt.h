@@ -47,6 +47,7 @@ struct StepSpec {
   int tile = -1;     // GEMM tile after resolve_step
   bool stream_ab = true;  // stream kernel: X = A (true) or X = B
   bool gemm_swap = false; // GEMM runs as C^T = B^T A^T (C's unit stride on the M side)
+  int arith = -1;         // complex64 GEMM: 0 FP32, 1 FP16 hi/lo; -1 = the context's
   int dtype = -1;         // PQ_C128 / PQ_C64; -1 = the context's
   uint64_t Mt = 1, Nt = 1, Kt = 1;
 };
@@ -76,6 +77,7 @@ struct pq_ctx {
   int override_kc = -1;
   int gemm_tile = -1;
   int gemm_orient = -1;
+  int arith = 0;
   void* scratch = nullptr;  // split-K partials for eager calls
   size_t scratch_bytes = 0;
 };

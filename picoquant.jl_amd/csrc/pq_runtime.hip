@@ -233,7 +233,8 @@ int resolve_step(pq_ctx* c, StepSpec& s, size_t esize) {
   // 16-wide tile and there is enough work per output.
   const bool gemm_ok = Kt <= (uint64_t)gemm_k_limit() && Kt >= 4 && std::max(Mt, Nt) >= 16 &&
                        Mt * Nt >= 128;
-  int kc = s.kc >= 0 ? s.kc : c->override_kc;
+  const bool step_kc = s.kc >= 0;   // a step's own class is taken as given
+  int kc = step_kc ? s.kc : c->override_kc;
   if (kc < 0) {
     if ((ab && Kt * Nt <= 64) || (ba && Kt * Mt <= 64)) kc = PQ_KC_STREAM;
     else if (Mt * Nt <= 16 && Kt > 64) kc = PQ_KC_REDUCE;
@@ -244,8 +245,16 @@ int resolve_step(pq_ctx* c, StepSpec& s, size_t esize) {
   if (kc == PQ_KC_STREAM) {
     if (!ab && !ba) return fail(c, PQ_ERR_UNSUPPORTED, "stream kernel not applicable");
     s.stream_ab = ab && (Mt >= Nt || !ba);
-  } else if (kc == PQ_KC_GEMM) {
+  } else if (kc == PQ_KC_GEMM || kc == PQ_KC_GEMM16) {
     if (!gemm_ok) return fail(c, PQ_ERR_UNSUPPORTED, "gemm kernel not applicable");
+    // FP16 hi/lo arithmetic: complex64 steps whose first K mode runs
+    // contiguously in both operands with an extent that is a multiple of 32.
+    const int arith = s.arith >= 0 ? s.arith : c->arith;
+    const bool k32 = !s.K.empty() && s.K[0].s0 == 1 && s.K[0].s1 == 1 && s.K[0].ext % 32 == 0;
+    const bool c64 = (s.dtype >= 0 ? s.dtype : c->dtype) == PQ_C64;
+    if (kc == PQ_KC_GEMM16 && !(c64 && k32))
+      return fail(c, PQ_ERR_UNSUPPORTED, "FP16 split GEMM needs complex64 and contiguous K of 32s");
+    if (kc == PQ_KC_GEMM && !step_kc && arith == 1 && c64 && k32) kc = PQ_KC_GEMM16;
     // The MFMA result layout puts 16 consecutive lanes on 16 consecutive
     // N-side outputs; when C's unit stride lies on the M side the product
     // is computed transposed so that those lanes write adjacent addresses.
@@ -255,7 +264,13 @@ int resolve_step(pq_ctx* c, StepSpec& s, size_t esize) {
     s.gemm_swap = c->gemm_orient >= 0 ? c->gemm_orient == 1 : (c_unit_m && !c_unit_n);
     int t = s.tile >= 0 ? s.tile : c->gemm_tile;
     if (t >= PQ_GEMM_TILES) return fail(c, PQ_ERR_ARG, "GEMM tile out of range");
-    s.tile = t >= 0 ? t : (s.gemm_swap ? pick_tile(Nt, Mt, esize) : pick_tile(Mt, Nt, esize));
+    const uint64_t m = s.gemm_swap ? Nt : Mt, n = s.gemm_swap ? Mt : Nt;
+    if (t >= 0)
+      s.tile = t;
+    else if (kc == PQ_KC_GEMM16)   // the split's conversions favour square 64x64 tiles
+      s.tile = n <= 16 ? 6 : (m <= 16 ? 9 : 3);
+    else
+      s.tile = pick_tile(m, n, esize);
   } else if (kc == PQ_KC_REDUCE) {
     if (Mt * Nt > 16) return fail(c, PQ_ERR_UNSUPPORTED, "reduce kernel needs M*N <= 16");
   } else if (kc != PQ_KC_GENERIC) {
@@ -297,7 +312,7 @@ static int launch_typed(pq_ctx* c, const C* pa, const C* pb, C* pc, const StepSp
       ProfScope ps(c, PQ_KC_STREAM, alg_bytes, alg_flops);
       e = launch_stream<C>(pb, pa, pc, sa, c->stream);
     }
-  } else if (s.kc == PQ_KC_GEMM) {
+  } else if (s.kc == PQ_KC_GEMM || s.kc == PQ_KC_GEMM16) {
     // Operand roles as launched: transposed product swaps A/B, M/N, the K
     // strides and the batch strides.
     const std::vector<Mode>& gM = s.gemm_swap ? s.N : s.M;
@@ -333,8 +348,15 @@ static int launch_typed(pq_ctx* c, const C* pa, const C* pb, C* pc, const StepSp
       for (int j = 0; j < L.nmodes && ok; ++j) ok = L.sA[j] % vec == 0 && L.sB[j] % vec == 0;
       ga.contig = ok ? 1 : 0;
     }
-    ProfScope ps(c, PQ_KC_GEMM, alg_bytes, alg_flops);
-    e = launch_gemm<C>(xa, xb, pc, ga, c->stream);
+    bool split = false;
+    if constexpr (sizeof(C) == 8) split = s.kc == PQ_KC_GEMM16 && gemm16_applicable(ga);
+    if (split) {
+      ProfScope ps(c, PQ_KC_GEMM16, alg_bytes, alg_flops);
+      if constexpr (sizeof(C) == 8) e = launch_gemm16(xa, xb, pc, ga, c->stream);
+    } else {
+      ProfScope ps(c, PQ_KC_GEMM, alg_bytes, alg_flops);
+      e = launch_gemm<C>(xa, xb, pc, ga, c->stream);
+    }
   } else {
     GenArgs ga;
     memset(&ga, 0, sizeof(ga));
@@ -723,6 +745,12 @@ int pq_profile_read(pq_ctx* c, int kc, int64_t* launches, double* total_ms, doub
 int pq_set_gemm_tile(pq_ctx* c, int tile) {
   if (!c || tile < -1 || tile >= PQ_GEMM_TILES) return PQ_ERR_ARG;
   c->gemm_tile = tile;
+  return PQ_OK;
+}
+
+int pq_set_arithmetic(pq_ctx* c, int mode) {
+  if (!c || mode < 0 || mode > 1) return PQ_ERR_ARG;
+  c->arith = mode;
   return PQ_OK;
 }
 

@@ -24,9 +24,9 @@ PQ_C64 = 1
 PQ_MAX_RANK = 64
 PQ_GEMM_TILES = 15
 
-KC_STREAM, KC_GEMM, KC_GENERIC, KC_REDUCE, KC_COPY, KC_BATCHED, KC_ACCUM = range(7)
+KC_STREAM, KC_GEMM, KC_GENERIC, KC_REDUCE, KC_COPY, KC_BATCHED, KC_ACCUM, KC_GEMM16 = range(8)
 KC_NAMES = {KC_STREAM: "stream", KC_GEMM: "gemm", KC_GENERIC: "generic", KC_REDUCE: "reduce",
-            KC_COPY: "copy", KC_BATCHED: "batched", KC_ACCUM: "accum"}
+            KC_COPY: "copy", KC_BATCHED: "batched", KC_ACCUM: "accum", KC_GEMM16: "gemm16"}
 
 # Every exported symbol, with (restype, argtypes).  tests/test_boundary.py
 # checks this list against include/pq_hip.h.
@@ -55,6 +55,7 @@ SIGNATURES = {
     "pq_set_kernel_override": (ct.c_int, [_P, ct.c_int]),
     "pq_set_gemm_tile": (ct.c_int, [_P, ct.c_int]),
     "pq_set_gemm_orientation": (ct.c_int, [_P, ct.c_int]),
+    "pq_set_arithmetic": (ct.c_int, [_P, ct.c_int]),
     "pq_plan_create": (ct.c_int, [_P, _P, ct.POINTER(_P)]),
     "pq_plan_execute": (ct.c_int, [_P, _P, ct.c_int]),
     "pq_plan_result": (ct.c_int, [_P, _P, ct.POINTER(ct.c_double)]),
@@ -214,6 +215,10 @@ class Context:
 
     def set_gemm_tile(self, tile):
         self.check(self.lib.pq_set_gemm_tile(self.ptr, int(tile)))
+
+    def set_arithmetic(self, mode):
+        """complex64 GEMM arithmetic: 0 FP32 matrix cores, 1 FP16 hi/lo split."""
+        self.check(self.lib.pq_set_arithmetic(self.ptr, int(mode)))
 
     def set_gemm_orientation(self, mode):
         """-1 automatic, 0 as given, 1 transposed (C^T = B^T A^T)."""

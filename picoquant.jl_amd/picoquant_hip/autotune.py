@@ -15,12 +15,15 @@ from .executor import SMALL_ELEMS, assign_precision, lower_plan
 TILES = {_lib.PQ_C64: list(range(15)), _lib.PQ_C128: list(range(11)) + [12, 14]}
 
 
-def tune_step(ctx, M, N, K, batch, dtype, reps=3, cache=None):
-    """(kclass, tile, us) of the fastest option for one contraction shape."""
-    key = (tuple(merge(M)), tuple(merge(N)), tuple(merge(K)), batch, dtype)
+def tune_step(ctx, M, N, K, batch, dtype, reps=3, cache=None, split16=False):
+    """(kclass, tile, us) of the fastest option for one contraction shape;
+    split16 adds the FP16 hi/lo GEMM to the options."""
+    key = (tuple(merge(M)), tuple(merge(N)), tuple(merge(K)), batch, dtype, split16)
     if cache is not None and key in cache:
         return cache[key]
     options = [(-1, -1), (_lib.KC_STREAM, -1)] + [(_lib.KC_GEMM, t) for t in TILES[dtype]]
+    if split16 and dtype == _lib.PQ_C64:
+        options += [(_lib.KC_GEMM16, t) for t in TILES[dtype]]
     best = None
     for kc, tile in options:
         try:
@@ -39,6 +42,9 @@ def autotune_plan(ctx, network, plan, sliced=(), group_bits=None, precision="mix
     """Per contraction step of `plan`: (kc list, tile list, report).  Small
     steps (grouped launches) keep the automatic choice."""
     slots, steps, _ = lower_plan(network, plan, sliced)
+    split16 = precision == "mixed16"
+    if split16:
+        precision = "mixed"
     code = _lib.PQ_C128 if (precision == "mixed" or ctx.code == _lib.PQ_C128) else _lib.PQ_C64
     dts, steps, slots = assign_precision(slots, steps, code, precision)
     n_sliced = len(list(sliced))
@@ -53,7 +59,7 @@ def autotune_plan(ctx, network, plan, sliced=(), group_bits=None, precision="mix
             tiles.append(-1)
             continue
         batch = (1 << gb) if slots[sc].dep else 1
-        best = tune_step(ctx, M, N, K, batch, dts[sc], reps, cache)
+        best = tune_step(ctx, M, N, K, batch, dts[sc], reps, cache, split16)
         if best is None:
             kcs.append(-1)
             tiles.append(-1)

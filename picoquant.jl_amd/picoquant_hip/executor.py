@@ -44,6 +44,7 @@ class PlanDesc(ct.Structure):
         ("out_slot", ct.c_int32),
         ("slot_dtype", ct.POINTER(ct.c_int32)),
         ("step_kind", ct.POINTER(ct.c_int32)),
+        ("step_arith", ct.POINTER(ct.c_int32)),
     ]
 
 
@@ -319,9 +320,11 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
                  budget_bytes=64 << 30, precision="uniform", fp64_steps=(), carry_elems=SMALL_ELEMS):
     """Compile a pair plan of a network whose backend is a HIPBackend.
 
-    precision: "uniform" (every step in the backend's type) or "mixed" (FP64
+    precision: "uniform" (every step in the backend's type), "mixed" (FP64
     backend: small steps in FP64, large ones in complex64 on FP32 MFMA; see
-    assign_precision; fp64_steps forces contraction steps to FP64).  kc / tiles, when given, are per contraction step of
+    assign_precision; fp64_steps forces contraction steps to FP64) or
+    "mixed16" (as "mixed", with the complex64 GEMMs on FP16 hi/lo matrix
+    cores where their shape allows).  kc / tiles, when given, are per contraction step of
     `plan` (kernel class and GEMM tile, -1 = automatic)."""
     backend = network.backend
     ctx = backend.ctx
@@ -331,9 +334,11 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
         raise KeyError(f"leaf tensors missing on the device: {missing[:4]}")
     slots, steps, out_slot = lower_plan(network, plan, sliced, out_order, handles, strides, carry_elems)
     n_sliced = len(list(sliced))
-    dts, steps, slots = assign_precision(slots, steps, ctx.code, precision, fp64_steps=fp64_steps)
+    split16 = precision == "mixed16"
+    dts, steps, slots = assign_precision(slots, steps, ctx.code, "mixed" if split16 else precision,
+                                         fp64_steps=fp64_steps)
     if group_bits is None:
-        esize = 8 if precision == "mixed" or ctx.code == _lib.PQ_C64 else 16
+        esize = 8 if precision in ("mixed", "mixed16") or ctx.code == _lib.PQ_C64 else 16
         group_bits = choose_group_bits(slots, n_sliced, budget_bytes, esize)
     slot_elems = [s.elems for s in slots]
     slot_batched = [1 if (s.dep and not s.leaf) else 0 for s in slots]
@@ -341,7 +346,7 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
     lss = []
     for s in slots:
         lss += (list(s.slice_stride) if s.slice_stride else [0] * n_sliced)
-    step_slots, nmodes, modes, kinds = [], [], [], []
+    step_slots, nmodes, modes, kinds, ariths = [], [], [], [], []
     kcs_in = list(kc) if kc is not None else None
     tls_in = list(tiles) if tiles is not None else None
     kcs, tls = [], []
@@ -350,6 +355,7 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
         step_slots += [sa, sb, sc]
         nmodes += [len(M), len(N), len(K)]
         kinds.append(kind)
+        ariths.append(1 if (split16 and kind == 0 and dts[sc] == _lib.PQ_C64) else 0)
         for grp in (M, N, K):
             for m in grp:
                 modes += list(m)
@@ -365,10 +371,10 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
         slot_leaf=_arr(ct.c_int64, slot_leaf), lss=_arr(ct.c_int64, lss or [0]),
         step_slots=_arr(ct.c_int32, step_slots), kc=_arr(ct.c_int32, kcs), tile=_arr(ct.c_int32, tls),
         nmodes=_arr(ct.c_int32, nmodes), modes=_arr(ct.c_int64, modes or [0]),
-        dt=_arr(ct.c_int32, dts), kind=_arr(ct.c_int32, kinds))
+        dt=_arr(ct.c_int32, dts), kind=_arr(ct.c_int32, kinds), arith=_arr(ct.c_int32, ariths))
     d = PlanDesc(len(slots), keep["slot_elems"], keep["slot_batched"], keep["slot_leaf"], n_sliced, keep["lss"],
                  group_bits, len(steps), keep["step_slots"], keep["kc"], keep["tile"], keep["nmodes"],
-                 keep["modes"], out_slot, keep["dt"], keep["kind"])
+                 keep["modes"], out_slot, keep["dt"], keep["kind"], keep["arith"])
     p = ct.c_void_p()
     ctx.check(ctx.lib.pq_plan_create(ctx.ptr, ct.cast(ct.pointer(d), ct.c_void_p), ct.byref(p)))
     return Plan(ctx, p, slots, steps, out_slot, n_sliced, group_bits, dts)

@@ -4,6 +4,7 @@ shape of the grid sweep plans.  Tolerances as in test_gpu_contract."""
 import numpy as np
 import pytest
 
+from oracle import pq_oracle as O
 from picoquant_hip import _lib
 from test_gpu_contract import check, rand
 
@@ -99,3 +100,62 @@ def test_boundary_times_site(ctx):
     check(ctx, A, ia, B, ib, got)
     for x in (ha, hb, hc):
         ctx.free(x)
+
+
+GEMM16_TOL = 1e-5    # relative to sum |a||b|: hi/lo split drops lo*lo (~2^-22) and rounds lo
+
+
+@pytest.mark.parametrize("orient", [0, 1], ids=["as-given", "transposed"])
+@pytest.mark.parametrize("tile", list(range(_lib.PQ_GEMM_TILES)))
+def test_gemm16_split_against_oracle(tile, orient):
+    """complex64 GEMM on FP16 hi/lo matrix cores (PQ_KC_GEMM16): K contiguous
+    in both operands and a multiple of 32; operands spanning many binades
+    (per-row scaling) and ragged M, N."""
+    ctx = _lib.Context(np.complex64)
+    try:
+        rng = np.random.default_rng(100 + tile)
+        M, N, K = 80, 72, 96
+        A = rand(rng, (K, M)) * np.exp2(rng.integers(-30, 30, size=(1, M)))      # row scales 2^+-30
+        B = rand(rng, (K, N)) * np.exp2(rng.integers(-20, 20, size=(1, N)))
+        A, B = A.astype(np.complex64), B.astype(np.complex64)
+        ha, hb = ctx.upload(A), ctx.upload(B)
+        ctx.set_kernel_override(_lib.KC_GEMM16)
+        ctx.set_gemm_tile(tile)
+        ctx.set_gemm_orientation(orient)
+        ctx.profile_enable(True)
+        try:
+            hc = ctx.contract(ha, [1, -1], hb, [1, -2])
+        finally:
+            ctx.set_kernel_override(-1)
+            ctx.set_gemm_tile(-1)
+            ctx.set_gemm_orientation(-1)
+        got = ctx.download(hc)
+        assert ctx.profile_read(_lib.KC_GEMM16)["launches"] == 1
+        ctx.profile_enable(False)
+        want = O.contract_tensors_data(A.astype(np.complex128), [1, -1], B.astype(np.complex128), [1, -2])
+        scale = O.contract_tensors_data(np.abs(A).astype(np.complex128), [1, -1],
+                                        np.abs(B).astype(np.complex128), [1, -2]).real
+        err = np.abs(got - want) / scale
+        assert err.max() <= GEMM16_TOL, err.max()
+        assert err.max() > 1e-9          # really ran in reduced precision
+    finally:
+        ctx.close()
+
+
+def test_gemm16_arithmetic_setting_and_fallback():
+    """pq_set_arithmetic(1) routes qualifying complex64 GEMMs to the split
+    kernel; a K that is not a multiple of 32 stays on FP32."""
+    ctx = _lib.Context(np.complex64)
+    try:
+        rng = np.random.default_rng(7)
+        ctx.set_arithmetic(1)
+        ctx.profile_enable(True)
+        for K, cls in ((64, _lib.KC_GEMM16), (40, _lib.KC_GEMM)):
+            A = rand(rng, (K, 96)).astype(np.complex64)
+            B = rand(rng, (K, 64)).astype(np.complex64)
+            check(ctx, A, [1, -1], B, [1, -2], gemm(ctx, A, [1, -1], B, [1, -2]))
+            assert ctx.profile_read(cls)["launches"] >= 1
+        with pytest.raises(_lib.PQError):
+            ctx.set_arithmetic(2)
+    finally:
+        ctx.close()
