@@ -69,6 +69,19 @@ class SlotInfo:
         self.elems = 1
 
 
+def _assign_strides(C):
+    """Column-major strides of an intermediate in its layout order; kept
+    sliced bonds last."""
+    st = 1
+    for x in C.idx:
+        C.stride[x] = st
+        st *= C.ext[x]
+    for j in C.carry:
+        C.slice_stride[j] = st
+        st *= 2
+    C.elems = st
+
+
 def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf_strides=None,
                carry_elems=SMALL_ELEMS):
     """Pure-host lowering (no device calls): returns (slots, steps, out_slot)
@@ -105,6 +118,7 @@ def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf
         s.elems = int(np.prod(dims)) if dims else 1
         label_slot[lab] = len(slots)
         slots.append(s)
+    n_leaves = len(slots)
     # Pass 1: index sets of every intermediate, in reference order.
     counter = network.counters["node"]
     raw = []
@@ -152,25 +166,27 @@ def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf
                 raise ValueError("output order does not match the open indices")
             C.idx = list(order)
         elif sc in consumer:
-            pslot = slots[consumer[sc]]
+            p = consumer[sc]
+            pslot = slots[p]
             cset = set(C.idx)
-            if consumer[sc] < sc:
+            pset = set(pslot.idx)
+            adapt = (p < sc and not pslot.leaf and p >= n_leaves and pslot.elems <= SMALL_ELEMS
+                     and pslot.elems < C.elems)
+            if p < sc and not adapt:
                 # partner already laid out: take the summed indices in its
                 # order, so K is contiguous in the same order in both operands
                 first = [x for x in pslot.idx if x in cset]
             else:
-                pset = set(pslot.idx)
+                # own order (this step's A-side indices before its B-side
+                # ones); a small partner laid out earlier is laid out again
+                # to match, as only this step reads it
                 first = [x for x in C.idx if x in pset]
             fset = set(first)
             C.idx = first + [x for x in C.idx if x not in fset]
-        st = 1
-        for x in C.idx:
-            C.stride[x] = st
-            st *= C.ext[x]
-        for j in C.carry:
-            C.slice_stride[j] = st
-            st *= 2
-        C.elems = st
+            if adapt:
+                pslot.idx = first + [x for x in pslot.idx if x not in fset]
+                _assign_strides(pslot)
+        _assign_strides(C)
     # Pass 3: modes.
     steps = []
     for sa, sb, sc in raw:
