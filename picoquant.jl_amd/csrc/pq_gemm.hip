@@ -382,7 +382,11 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
 // K-tile depth of the fast path: 32 for complex64 tiles whose LDS allows it.
 template <typename C, int BM, int BN>
 constexpr int fast_bk() {
-  return (sizeof(C) == 8 && (BM + BN) * 33 * 8 + 16 * (BM + BN) <= 60 * 1024) ? 32 : 16;
+  // 32 when both tiles at pitch 33 plus the offset tables fit 80 KiB (two
+  // workgroups per CU's 160 KiB); 64-deep tiles measured slower (their
+  // staging registers halve the occupancy)
+  constexpr size_t es = sizeof(C), tab = 16 * (BM + BN) + 2 * 4 * 2048 / 8;
+  return (BM + BN + 2) * 33 * es + tab <= 80 * 1024 ? 32 : 16;
 }
 
 // Static LDS stays within 64 KiB: complex128 tiles up to 128 rows or
@@ -399,8 +403,9 @@ hipError_t launch_tile(const C* A, const C* B, C* Out, const GemmArgs& g, hipStr
   } else {
     dim3 grid((g.M.total + BM - 1) / BM, (g.N.total + BN - 1) / BN, g.L.total);
     constexpr int BKF = fast_bk<C, BM, BN>();
-    // K tile depth: the deepest of 32 / 16 / 8 that divides the first
-    // (contiguous) K mode, so a tile never straddles two runs.
+    // K tile depth: the deepest of 32 / 16 / 8 (within the tile's LDS
+    // budget) that divides the first (contiguous) K mode, so a tile never
+    // straddles two runs.
     const uint32_t k0ext = g.K.nmodes > 0 ? g.K.ext[0].d : 0;
     if (g.contig && k0ext > 0) {
       if (k0ext % BKF == 0) {
