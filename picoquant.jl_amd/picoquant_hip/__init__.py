@@ -14,6 +14,8 @@ from .contraction import (contract_network, contract_pair, create_ncon_indices,
 from .slicing import (multi_index_partition, partition_network_on_virtual_bonds,
                       replace_with_view, slice_tensor_network)
 from .optimal_order import netcon, netcon_contraction
+from .network import network_from_json, to_json
+from .qasm import convert_qiskit_circ_to_network, load_qasm_as_circuit, load_qasm_as_circuit_from_file
 from . import circuits
 
 __all__ = [
@@ -23,5 +25,6 @@ __all__ = [
     "contract_network", "contract_pair", "create_ncon_indices", "full_wavefunction_contraction",
     "random_contraction_plan", "sort_indices", "multi_index_partition",
     "partition_network_on_virtual_bonds", "replace_with_view", "slice_tensor_network", "circuits",
-    "netcon", "netcon_contraction",
+    "netcon", "netcon_contraction", "network_from_json", "to_json", "convert_qiskit_circ_to_network",
+    "load_qasm_as_circuit", "load_qasm_as_circuit_from_file",
 ]

@@ -7,6 +7,8 @@ save_tensor_data exactly where the reference calls it.
 """
 from __future__ import annotations
 
+import json
+
 import math
 from collections import OrderedDict
 from dataclasses import dataclass, field
@@ -245,3 +247,70 @@ def gate_tensor(gate: str):
     if gate not in GATE_TENSORS:
         raise KeyError(f"Invalid input to gate_tensor: {gate}")
     return GATE_TENSORS[gate].copy()
+
+
+# -- serialisation (src/layer3.jl:578-707) -----------------------------------
+# The structure only (nodes, edges, qubit bookkeeping), in the reference's
+# schema, so files written by either side load in the other; tensor data
+# stays in the backend under each node's data_label.
+
+def edge_to_dict(edge):
+    """to_dict(edge::Edge) (src/layer3.jl:584-587)."""
+    return {"src": edge.src, "dst": edge.dst, "virtual": edge.virtual, "qubit": edge.qubit}
+
+
+def edge_from_dict(d):
+    """src/layer3.jl:594-599."""
+    return Edge(d["src"], d["dst"], d["qubit"], d["virtual"])
+
+
+def node_to_dict(node):
+    """to_dict(node::Node) (src/layer3.jl:606-611): dims as strings."""
+    return {"indices": list(node.indices), "dims": [str(x) for x in node.dims], "data_label": node.data_label}
+
+
+def node_from_dict(d):
+    """src/layer3.jl:618-623."""
+    return Node(list(d["indices"]), [int(x) for x in d["dims"]], d["data_label"])
+
+
+def to_dict(network):
+    """to_dict(network) (src/layer3.jl:630-647)."""
+    return OrderedDict([
+        ("number_qubits", network.number_qubits),
+        ("edges", OrderedDict((k, edge_to_dict(v)) for k, v in network.edges.items())),
+        ("nodes", OrderedDict((k, node_to_dict(v)) for k, v in network.nodes.items())),
+        ("input_qubits", list(network.input_qubits)),
+        ("output_qubits", list(network.output_qubits)),
+        ("qubit_ordering", list(network.qubit_ordering)),
+        ("node_layers", OrderedDict((k, v) for k, v in network.node_layers.items())),
+    ])
+
+
+def network_from_dict(d, backend=None):
+    """network_from_dict (src/layer3.jl:654-685): counters from the largest
+    index / node numbers, backend tensors not touched."""
+    backend = backend if backend is not None else HIPBackend()
+    counters = {"index": 0, "node": 0, "layer": 0}
+    edges = OrderedDict()
+    for k, v in d["edges"].items():
+        counters["index"] = max(counters["index"], int(k.split("_")[-1]))
+        edges[k] = edge_from_dict(v)
+    nodes = OrderedDict()
+    for k, v in d["nodes"].items():
+        counters["node"] = max(counters["node"], int(k.split("_")[-1]))
+        nodes[k] = node_from_dict(v)
+    layers = {k: v for k, v in d["node_layers"].items()}
+    counters["layer"] = max(layers.values(), default=0)
+    return TensorNetworkCircuit(backend, d["number_qubits"], list(d["input_qubits"]), list(d["output_qubits"]),
+                                nodes, edges, list(d["qubit_ordering"]), counters, layers)
+
+
+def to_json(network, indent=0):
+    """to_json (src/layer3.jl:690-696)."""
+    return json.dumps(to_dict(network), indent=indent or None)
+
+
+def network_from_json(json_str, backend=None):
+    """network_from_json (src/layer3.jl:704-707)."""
+    return network_from_dict(json.loads(json_str, object_pairs_hook=OrderedDict), backend)

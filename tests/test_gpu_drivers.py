@@ -200,3 +200,17 @@ def test_netcon_contraction(dtype):
     sv = O.statevector(9, O.grid_circuit_gates(3, 3, 1, 7))
     want = sv[sum(int(b) << q for q, b in enumerate(bits))]
     assert rel_err([amp], [want]) < RTOL[dtype]
+
+
+def test_qasm_qft10_on_gpu(golden_dir, dtype):
+    """examples/qft_10.qasm (qiskit-transpiled u1/u2/cx) through the QASM
+    parser and full_wavefunction_contraction! on HIPBackend: the DFT."""
+    from picoquant_hip import qasm as Q
+    circ = Q.load_qasm_as_circuit_from_file(golden_dir / "qft_10.qasm")
+    tn = Q.convert_qiskit_circ_to_network(circ, P.HIPBackend(dtype))
+    x = 613
+    P.add_input(tn, "".join(str((x >> q) & 1) for q in range(10)))
+    P.full_wavefunction_contraction(tn, "vector")
+    psi = tn.backend.load_tensor_data("result")
+    y = np.arange(1024)
+    assert rel_err(psi, np.exp(2j * np.pi * x * y / 1024) / 32) < RTOL[dtype]
