@@ -16,6 +16,7 @@ from .slicing import (multi_index_partition, partition_network_on_virtual_bonds,
 from .optimal_order import netcon, netcon_contraction
 from .network import network_from_json, to_json
 from .qasm import convert_qiskit_circ_to_network, load_qasm_as_circuit, load_qasm_as_circuit_from_file
+from .dsl import DSLBackend, execute_dsl_file
 from . import circuits
 
 __all__ = [
@@ -26,5 +27,5 @@ __all__ = [
     "random_contraction_plan", "sort_indices", "multi_index_partition",
     "partition_network_on_virtual_bonds", "replace_with_view", "slice_tensor_network", "circuits",
     "netcon", "netcon_contraction", "network_from_json", "to_json", "convert_qiskit_circ_to_network",
-    "load_qasm_as_circuit", "load_qasm_as_circuit_from_file",
+    "load_qasm_as_circuit", "load_qasm_as_circuit_from_file", "DSLBackend", "execute_dsl_file",
 ]

@@ -214,3 +214,28 @@ def test_qasm_qft10_on_gpu(golden_dir, dtype):
     psi = tn.backend.load_tensor_data("result")
     y = np.arange(1024)
     assert rel_err(psi, np.exp(2j * np.pi * x * y / 1024) / 32) < RTOL[dtype]
+
+
+def test_dsl_replay_on_gpu(tmp_path, dtype):
+    """A contraction recorded by DSLBackend and replayed by execute_dsl_file
+    through libpq_hip: the reference's DSL test amplitude (1/sqrt 2,
+    test/layer1_tests.jl:11-44) and a QFT-6 wavefunction."""
+    from picoquant_hip import dsl as D
+    tl, store = tmp_path / "contract_network.tl", tmp_path / "tensor_data.npz"
+    be = D.DSLBackend(tl, store, "", True)
+    tn = P.network_from_gates(3, [(C.H, [0]), (C.CX, [0, 1]), (C.CX, [1, 2])], backend=be)
+    P.add_input(tn, "000")
+    P.add_output(tn, "000")
+    P.contract_network(tn, P.random_contraction_plan(tn, np.random.default_rng(1)))
+    assert D.execute_dsl_file(dtype, tl, store) == [f"node_{tn.counters['node']}"]   # the final node
+    amp = complex(np.asarray(be.load_tensor_data("result")).reshape(-1)[0])
+    assert abs(amp - 1 / math.sqrt(2)) < RTOL[dtype]
+    tl2, store2, out2 = tmp_path / "qft.tl", tmp_path / "qft.npz", tmp_path / "out.npz"
+    be = D.DSLBackend(tl2, store2, out2, True)
+    n, x = 6, 45
+    tn = P.network_from_gates(n, C.qft_gates(n), backend=be)
+    P.add_input(tn, "".join(str((x >> q) & 1) for q in range(n)))
+    P.full_wavefunction_contraction(tn, "vector")
+    D.execute_dsl_file(dtype, tl2, store2)
+    psi = be.load_tensor_data("result")
+    assert rel_err(psi, O.qft_closed_form(n, x)) < RTOL[dtype]
