@@ -148,7 +148,10 @@ typedef struct pq_plan pq_plan;
 typedef struct {
   int32_t n_slots;
   const int64_t* slot_elems;        /* per-slice elements (leaves: all elements) */
-  const int32_t* slot_batched;      /* 1 if the slot depends on a sliced bond */
+  const int64_t* slot_batch_mask;   /* bit j set: the slot holds one copy per value of sliced
+                                       bond j within a slice group (for j < group_bits) and is
+                                       recomputed per slice group (j >= group_bits); 0 = the
+                                       slot does not depend on any sliced bond */
   const pq_tensor* slot_leaf;       /* leaf tensor, 0 for intermediates */
   int32_t n_sliced;                 /* number of sliced bonds (extent 2 each) */
   const int64_t* slot_slice_stride; /* [n_slots * n_sliced] stride of sliced bond j in an unbatched

@@ -66,7 +66,10 @@ size_t esize_of(int dtype) { return dtype == PQ_C128 ? 16 : 8; }
 
 struct Slot {
   int64_t elems = 0;      // per batch entry
-  bool batched = false;
+  uint64_t mask = 0;      // sliced bonds the slot depends on
+  uint64_t low = 0;       // ... of them within a slice group: one copy per value
+  bool batched = false;   // mask != 0
+  int64_t copies() const { return (int64_t)1 << __builtin_popcountll(low); }
   int dtype = PQ_C128;
   pq_tensor leaf = 0;
   std::vector<int64_t> slice_stride;  // unbatched slots: stride of each sliced bond kept (0 = absent)
@@ -78,7 +81,8 @@ struct Slot {
 struct PStep {
   int kind = 0;            // 0 contract, 1 convert
   int a, b, c;
-  bool dep;
+  bool dep;                // depends on a sliced bond (batched / selects per slice)
+  bool phase2 = false;     // depends on a bond above the slice group: rerun per group
   StepSpec spec;           // resolved; batch strides for phase 2 steps
   double flops = 0, bytes = 0;
   int level = 0;           // 1 + max level of the producers of A and B
@@ -155,7 +159,6 @@ int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
   unsigned g = (unsigned)std::min<int64_t>((nb + 255) / 256, 1024);
   hipLaunchKernelGGL(k_fill_zero, dim3(std::max(1u, g)), dim3(256), 0, c->stream, p->out_dev, 2 * nb);
   if (hipGetLastError() != hipSuccess) return fail(c, PQ_ERR_HIP, "fill kernel");
-  const int G = 1 << p->group_bits;
   auto do_launch = [&](const Launch& L, int64_t group) -> int {
     if (!L.grouped) {
       const int i = L.step;
@@ -165,7 +168,7 @@ int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
       if (st.kind == 1) {
         const Slot& S = p->slots[st.a];
         const Slot& D = p->slots[st.c];
-        int64_t n = D.elems * (D.batched ? G : 1);
+        int64_t n = D.elems * D.copies();
         unsigned gc = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
         const void* src = slot_ptr(c, p, st.a, group);
         void* dst = slot_ptr(c, p, st.c, group);
@@ -219,7 +222,7 @@ int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
     if (r) return r;
   }
   const Slot& os = p->slots[p->out_slot];
-  int nbatch = os.batched ? G : 1;
+  int nbatch = (int)os.copies();
   for (int64_t grp = 0; grp < p->n_groups; ++grp) {
     for (const Launch& L : p->seq2) {
       int r = do_launch(L, grp);
@@ -252,7 +255,6 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
   p->group_bits = d->group_bits;
   p->n_groups = (int64_t)1 << (d->n_sliced - d->group_bits);
   p->out_slot = d->out_slot;
-  const int G = 1 << d->group_bits;
   auto bad = [&](const std::string& m) {
     free_plan(p);
     return fail(c, PQ_ERR_ARG, m);
@@ -261,7 +263,11 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
   for (int s = 0; s < d->n_slots; ++s) {
     Slot& sl = p->slots[s];
     sl.elems = d->slot_elems[s];
-    sl.batched = d->slot_batched[s] != 0;
+    const uint64_t m = d->slot_batch_mask ? (uint64_t)d->slot_batch_mask[s] : 0;
+    if (d->n_sliced < 64 && (m >> d->n_sliced) != 0) return bad("batch mask names a bond that is not sliced");
+    sl.mask = m;
+    sl.low = m & (((uint64_t)1 << d->group_bits) - 1);
+    sl.batched = m != 0;
     sl.leaf = d->slot_leaf[s];
     sl.dtype = d->slot_dtype ? d->slot_dtype[s] : c->dtype;
     if (sl.dtype != PQ_C128 && sl.dtype != PQ_C64) return bad("bad slot element type");
@@ -326,14 +332,15 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
         if (m.ext < 1 || m.s0 < 0 || m.s1 < 0) return bad("bad mode");
     const Slot &A = p->slots[st.a], &B = p->slots[st.b], &Cs = p->slots[st.c];
     st.dep = Cs.batched;
-    if (!st.dep && (A.batched || B.batched)) return bad("unbatched output from a batched operand");
+    st.phase2 = (Cs.mask >> d->group_bits) != 0;
+    if ((A.mask | B.mask) & ~Cs.mask) return bad("result independent of a sliced bond its operand depends on");
     if (st.kind == 1) {
-      if (A.leaf || A.elems != Cs.elems || A.batched != Cs.batched || A.slice_stride != Cs.slice_stride)
+      if (A.leaf || A.elems != Cs.elems || A.mask != Cs.mask || A.slice_stride != Cs.slice_stride)
         return bad("convert needs a same-shaped intermediate");
-      const double n = (double)Cs.elems * (Cs.batched ? (double)G * p->n_groups : 1.0);
+      const double n = (double)Cs.elems * (double)Cs.copies() * (st.phase2 ? (double)p->n_groups : 1.0);
       st.bytes = n * (esize_of(A.dtype) + esize_of(Cs.dtype));
       p->alg_bytes += st.bytes;
-      (st.dep ? p->order2 : p->order1).push_back(i);
+      (st.phase2 ? p->order2 : p->order1).push_back(i);
       continue;
     }
     if (A.dtype != Cs.dtype || B.dtype != Cs.dtype) return bad("contraction operands differ in element type");
@@ -342,8 +349,8 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     // also reads slots that keep sliced bonds at every slice offset.
     auto fits = [&](const Slot& sl, int64_t span) -> bool {
       int64_t extra = 0;
-      if (st.dep)
-        for (int64_t v : sl.slice_stride) extra += v;
+      for (int j = 0; j < d->n_sliced; ++j)
+        if ((Cs.mask >> j) & 1) extra += sl.slice_stride[j];
       if (!sl.leaf) return span + extra < sl.elems;
       Tensor* t = get(c, sl.leaf);
       int64_t full = 0;
@@ -354,20 +361,48 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     if (!fits(B, span_of(st.spec.N, true) + span_of(st.spec.K, false))) return bad("B access out of range");
     if (!fits(Cs, span_of(st.spec.M, false) + span_of(st.spec.N, false))) return bad("C access out of range");
     // Batch group over the slices of a group.
+    // Batch group: one entry per value of the result's sliced bonds within
+    // the slice group.  A batched operand is read at its own copy for those
+    // of the bonds it depends on (stride 0 for the others); a slot that
+    // keeps a bond as an index is read at the bond's stride.
     st.spec.L = single_batch();
-    if (st.dep && G > 1) {
+    if (Cs.low) {
       Batch& L = st.spec.L;
-      L.nmodes = d->group_bits;
-      L.total = (uint32_t)G;
       auto stride_of = [&](const Slot& sl, int j) -> int64_t {
-        return sl.batched ? sl.elems * ((int64_t)1 << j) : sl.slice_stride[j];
+        if ((sl.low >> j) & 1)
+          return sl.elems << __builtin_popcountll(sl.low & (((uint64_t)1 << j) - 1));
+        return sl.slice_stride[j];
       };
+      L.nmodes = 0;
       for (int j = 0; j < d->group_bits; ++j) {
-        L.ext[j] = make_fdiv(2);
-        L.sA[j] = stride_of(A, j);
-        L.sB[j] = stride_of(B, j);
-        L.sC[j] = stride_of(Cs, j);
+        if (!((Cs.low >> j) & 1)) continue;
+        const int q = L.nmodes++;
+        L.ext[q] = make_fdiv(2);
+        L.sA[q] = stride_of(A, j);
+        L.sB[q] = stride_of(B, j);
+        L.sC[q] = stride_of(Cs, j);
       }
+      L.total = (uint32_t)Cs.copies();
+      // every batch entry stays inside its operand's copies
+      int64_t offA = 0, offB = 0, offC = 0;
+      for (int q = 0; q < L.nmodes; ++q) {
+        offA += L.sA[q];
+        offB += L.sB[q];
+        offC += L.sC[q];
+      }
+      auto inside = [&](const Slot& sl, int64_t off, int64_t span) -> bool {
+        if (sl.leaf) {
+          Tensor* t = get(c, sl.leaf);
+          int64_t full = 0;
+          for (int q = 0; q < t->rank; ++q) full += (t->dims[q] - 1) * t->strides[q];
+          return off + span <= full;
+        }
+        return off + span < sl.elems * sl.copies();
+      };
+      if (!inside(A, offA, span_of(st.spec.M, true) + span_of(st.spec.K, true)) ||
+          !inside(B, offB, span_of(st.spec.N, true) + span_of(st.spec.K, false)) ||
+          !inside(Cs, offC, span_of(st.spec.M, false) + span_of(st.spec.N, false)))
+        return bad("batch entries out of range");
     }
     st.spec.arith = d->step_arith ? d->step_arith[i] : 0;
     if (st.spec.arith < 0 || st.spec.arith > 1) return bad("bad step arithmetic");
@@ -379,17 +414,18 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
       free_plan(p);
       return fail(c, r, "step " + std::to_string(i) + ": " + msg);
     }
-    const double nbat = st.dep ? (double)G * p->n_groups : 1.0;
+    const double nbat = (double)Cs.copies() * (st.phase2 ? (double)p->n_groups : 1.0);
     const double es = (double)esize_of(st.spec.dtype);
     st.flops = 8.0 * st.spec.Mt * st.spec.Nt * st.spec.Kt * nbat;
     st.bytes = es * (st.spec.Mt * st.spec.Kt + st.spec.Kt * st.spec.Nt + st.spec.Mt * st.spec.Nt) * nbat;
     p->alg_flops += st.flops;
     p->alg_bytes += st.bytes;
-    (st.dep ? p->order2 : p->order1).push_back(i);
+    (st.phase2 ? p->order2 : p->order1).push_back(i);
     p->scratch_bytes = std::max(p->scratch_bytes, step_scratch_bytes(st.spec, esize_of(st.spec.dtype)));
   }
   if (!p->slots[p->out_slot].leaf && produced[p->out_slot] < 0) return bad("output slot never produced");
-  if (d->n_sliced > 0 && !p->slots[p->out_slot].batched) return bad("sliced plan with a slice-independent output");
+  if (d->n_sliced > 0 && p->slots[p->out_slot].mask != ((d->n_sliced >= 64) ? ~0ull : (((uint64_t)1 << d->n_sliced) - 1)))
+    return bad("sliced plan whose output does not depend on every sliced bond");
   // Launch sequences: by dependency level; the small contraction steps of a
   // level and element type share one grouped launch, the others launch alone.
   auto small = [&](const PStep& st) {
@@ -467,7 +503,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     for (int i : p->order2)
       if (p->steps[i].a == s || p->steps[i].b == s) read_in_2 = true;
     if (read_in_2 && start < phase2_start) stop = end;
-    size_t bytes = (size_t)sl.elems * esize_of(sl.dtype) * (sl.batched ? G : 1);
+    size_t bytes = (size_t)sl.elems * esize_of(sl.dtype) * (size_t)sl.copies();
     bytes = (bytes + 255) / 256 * 256;
     lives.push_back({s, start, stop, bytes});
   }
@@ -528,7 +564,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
           return fail(c, PQ_ERR_UNSUPPORTED, "too many modes in a grouped step");
         }
         m.L = st.spec.L;
-        m.nhigh = st.dep ? d->n_sliced - d->group_bits : 0;
+        m.nhigh = st.phase2 ? d->n_sliced - d->group_bits : 0;
         for (int j = 0; j < m.nhigh; ++j) {
           const int bit = d->group_bits + j;
           m.hA[j] = p->slots[st.a].slice_stride[bit];

@@ -58,7 +58,7 @@ def autotune_plan(ctx, network, plan, sliced=(), group_bits=None, precision="mix
             kcs.append(-1)
             tiles.append(-1)
             continue
-        batch = (1 << gb) if slots[sc].dep else 1
+        batch = 1 << sum(1 for j in slots[sc].deps if j < gb)
         best = tune_step(ctx, M, N, K, batch, dts[sc], reps, cache, split16)
         if best is None:
             kcs.append(-1)

@@ -70,7 +70,7 @@ def plan_samples(network, plan, sliced=(), group_bits=None, precision="mixed", m
             continue
         if max(slots[x].elems for x in (sa, sb, sc)) > max_elems:
             continue
-        batch = (1 << gb) if slots[sc].dep else 1
+        batch = 1 << sum(1 for j in slots[sc].deps if j < gb)
         if batch * slots[sc].elems > 4 * max_elems:
             batch = max(1, (4 * max_elems) // slots[sc].elems)
         out.append((step_features(M, N, K, batch, dts[sc]), (M, N, K, batch, dts[sc])))

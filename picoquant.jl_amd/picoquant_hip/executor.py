@@ -30,7 +30,7 @@ class PlanDesc(ct.Structure):
     _fields_ = [
         ("n_slots", ct.c_int32),
         ("slot_elems", ct.POINTER(ct.c_int64)),
-        ("slot_batched", ct.POINTER(ct.c_int32)),
+        ("slot_batch_mask", ct.POINTER(ct.c_int64)),
         ("slot_leaf", ct.POINTER(ct.c_int64)),
         ("n_sliced", ct.c_int32),
         ("slot_slice_stride", ct.POINTER(ct.c_int64)),
@@ -56,17 +56,22 @@ SMALL_ELEMS = 1 << 16   # steps up to this size run in FP64 under precision="mix
 
 
 class SlotInfo:
-    __slots__ = ("idx", "ext", "stride", "leaf", "dep", "slice_stride", "carry", "elems")
+    __slots__ = ("idx", "ext", "stride", "leaf", "deps", "slice_stride", "carry", "elems")
 
     def __init__(self):
         self.idx = []          # per-slice indices, in layout order
         self.ext = {}          # index -> extent
         self.stride = {}       # index -> per-slice stride
         self.leaf = 0
-        self.dep = False       # batched over the slices of a slice group
+        self.deps = []         # sliced bonds whose values select a copy of this slot (batched)
         self.slice_stride = [] # per sliced bond: its stride when kept as an index, else 0
         self.carry = []        # intermediates: sliced bonds kept as indices (laid out last)
         self.elems = 1
+
+    @property
+    def dep(self):
+        """Depends on (is batched over) at least one sliced bond."""
+        return bool(self.deps)
 
 
 def _assign_strides(C):
@@ -119,6 +124,7 @@ def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf
         label_slot[lab] = len(slots)
         slots.append(s)
     n_leaves = len(slots)
+    full_deps = {}
     # Pass 1: index sets of every intermediate, in reference order.
     counter = network.counters["node"]
     raw = []
@@ -135,10 +141,13 @@ def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf
         ca = set(A.carry) | {j for j, v in enumerate(A.slice_stride) if v}
         cb = set(B.carry) | {j for j, v in enumerate(B.slice_stride) if v}
         size = math.prod(C.ext.values()) << len(ca | cb)
+        # a result depends on the sliced bonds its operands depend on or
+        # keep as indices -- only those select its copies (partial batching)
+        full_deps[len(slots)] = sorted(set(A.deps) | set(B.deps) | ca | cb)
         if (ca or cb) and not (A.dep or B.dep or ca & cb) and max(size, A.elems, B.elems) <= carry_elems:
             C.carry = sorted(ca | cb)
         else:
-            C.dep = A.dep or B.dep or bool(ca or cb)
+            C.deps = full_deps[len(slots)]
         C.slice_stride = [0] * nsl
         C.elems = size if C.carry else math.prod(C.ext.values())
         counter += 1
@@ -151,7 +160,9 @@ def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf
     if slots[out_slot].carry:
         # the result is per slice (summed over slices afterwards)
         slots[out_slot].carry = []
-        slots[out_slot].dep = True
+        slots[out_slot].deps = full_deps[out_slot]
+    if nsl and sorted(slots[out_slot].deps) != list(range(nsl)):
+        raise ValueError("the result does not depend on every sliced bond")
     # Pass 2: layouts.  Consumer of each intermediate -> indices summed there.
     consumer = {}
     for sa, sb, sc in raw:
@@ -239,7 +250,7 @@ def assign_precision(slots, steps, ctx_code, precision, small_elems=SMALL_ELEMS,
                 src = slots[x]
                 y = SlotInfo()
                 y.idx, y.ext, y.stride = list(src.idx), dict(src.ext), dict(src.stride)
-                y.dep, y.elems = src.dep, src.elems
+                y.deps, y.elems = list(src.deps), src.elems
                 y.slice_stride, y.carry = list(src.slice_stride), list(src.carry)
                 slots.append(y)
                 dt.append(want)
@@ -324,10 +335,14 @@ def _leaf_layout(backend):
 
 
 def choose_group_bits(slots, n_sliced, budget_bytes, esize):
-    """Largest slice group whose batched intermediates fit the budget."""
-    peak = max((s.elems for s in slots if s.dep and not s.leaf), default=1)
+    """Largest slice group whose batched intermediates fit the budget (a slot
+    holds one copy per value of its sliced bonds within the group)."""
     gb = n_sliced
-    while gb > 0 and 3 * peak * esize * (1 << gb) > budget_bytes:
+
+    def peak(g):
+        return max((s.elems << sum(1 for j in s.deps if j < g) for s in slots if s.dep and not s.leaf), default=1)
+
+    while gb > 0 and 3 * peak(gb) * esize > budget_bytes:
         gb -= 1
     return min(gb, 16)
 
@@ -357,7 +372,7 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
         esize = 8 if precision in ("mixed", "mixed16") or ctx.code == _lib.PQ_C64 else 16
         group_bits = choose_group_bits(slots, n_sliced, budget_bytes, esize)
     slot_elems = [s.elems for s in slots]
-    slot_batched = [1 if (s.dep and not s.leaf) else 0 for s in slots]
+    slot_mask = [0 if s.leaf else sum(1 << j for j in s.deps) for s in slots]
     slot_leaf = [s.leaf for s in slots]
     lss = []
     for s in slots:
@@ -383,12 +398,12 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
             kcs.append(-1)
             tls.append(-1)
     keep = dict(
-        slot_elems=_arr(ct.c_int64, slot_elems), slot_batched=_arr(ct.c_int32, slot_batched),
+        slot_elems=_arr(ct.c_int64, slot_elems), slot_mask=_arr(ct.c_int64, slot_mask),
         slot_leaf=_arr(ct.c_int64, slot_leaf), lss=_arr(ct.c_int64, lss or [0]),
         step_slots=_arr(ct.c_int32, step_slots), kc=_arr(ct.c_int32, kcs), tile=_arr(ct.c_int32, tls),
         nmodes=_arr(ct.c_int32, nmodes), modes=_arr(ct.c_int64, modes or [0]),
         dt=_arr(ct.c_int32, dts), kind=_arr(ct.c_int32, kinds), arith=_arr(ct.c_int32, ariths))
-    d = PlanDesc(len(slots), keep["slot_elems"], keep["slot_batched"], keep["slot_leaf"], n_sliced, keep["lss"],
+    d = PlanDesc(len(slots), keep["slot_elems"], keep["slot_mask"], keep["slot_leaf"], n_sliced, keep["lss"],
                  group_bits, len(steps), keep["step_slots"], keep["kc"], keep["tile"], keep["nmodes"],
                  keep["modes"], out_slot, keep["dt"], keep["kind"], keep["arith"])
     p = ct.c_void_p()

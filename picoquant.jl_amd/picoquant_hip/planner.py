@@ -307,16 +307,18 @@ ROOFLINE = {"flops": 100e12, "bytes": 4e12, "launch": 4e-6, "esize": 8}
 
 
 def sliced_cost(steps, ext, smask, nslices, model="macs"):
-    """Total cost over all slices (steps that do not depend on a sliced bond
-    run once) and the largest per-slice intermediate.  model="macs": complex
-    multiply-adds; model="time": roofline seconds (ROOFLINE)."""
+    """Total cost over all slices and the largest per-slice intermediate.  A
+    step runs once per value of the sliced bonds in its subtree (its result
+    depends on those only; steps without any run once).  model="macs":
+    complex multiply-adds; model="time": roofline seconds (ROOFLINE).
+    nslices is kept for the callers' signature (= 2^popcount(smask))."""
     total = 0
     peak = 0
     R = ROOFLINE
     for c, k, sub, ma, mb in steps:
         size_c = _mask_size(c & ~smask, ext)
         size_k = _mask_size(k & ~smask, ext)
-        mult = nslices if (sub & smask) else 1
+        mult = 1 << bin(sub & smask).count("1")
         if model == "macs":
             total += size_c * size_k * mult
         else:
@@ -372,12 +374,12 @@ def _sliced_costs(mats, chosen, nslices, model):
     bk = inK @ ch
     ba = inA @ ch
     bb = inB @ ch
-    bs = (inS @ ch) > 0
+    bs = inS @ ch
     # every candidate bond has extent 2: each sliced bond halves a tensor
     c = lc[:, None] - bc[:, None] - inC
     k = lk[:, None] - bk[:, None] - inK
-    dep = bs[:, None] | inS
-    mult = np.where(dep, float(nslices), 1.0)
+    # a step runs once per value of the sliced bonds in its subtree
+    mult = np.exp2(bs[:, None] + inS)
     if model == "macs":
         cost = (np.exp2(c + k) * mult).sum(0)
     else:

@@ -109,13 +109,14 @@ def test_slicer_cost_matches_recount():
     plan = PL.sweep_plan(tn, PL.grid_site_order(3, 3))
     bonds, tot, peak = PL.choose_slices(tn, plan, 4)
     assert len(set(bonds)) == 4
-    # recount from the lowered plan: dependent steps run once per slice
-    # (the slicer's model batches every step that touches a sliced bond)
+    # recount from the lowered plan: a step runs once per value of the
+    # sliced bonds its result depends on (without carrying small steps, as
+    # the slicer's model)
     slots, steps, _ = lower_plan(tn, plan, bonds, carry_elems=0)
     recount = 0
     for sa, sb, sc, M, N, K in steps:
         m = math.prod(x[0] for x in M) * math.prod(x[0] for x in N) * math.prod(x[0] for x in K)
-        recount += m * (16 if slots[sc].dep else 1)
+        recount += m * 2 ** len(slots[sc].deps)
     assert recount == tot
     unsliced = PL.plan_ledger(tn, plan)["macs"]
     assert tot >= unsliced
