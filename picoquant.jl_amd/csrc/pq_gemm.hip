@@ -45,6 +45,19 @@ template <> struct mfma_t<double> {
 constexpr int BK = 16;
 constexpr int KTAB_MAX = 2048;   // K tables held in LDS
 
+// Workgroups are dealt to the 8 XCDs round-robin in launch order, and each
+// XCD has its own L2.  Renumber the (x, y) tiles of a batch entry so that
+// each XCD gets one contiguous run of tiles: the tiles that share rows of
+// the large operand (consecutive x for one y) then meet in the same L2.
+__device__ __forceinline__ void xcd_tile(uint32_t& tx, uint32_t& ty) {
+  const uint32_t gx = gridDim.x, n = gridDim.x * gridDim.y;
+  const uint32_t b = blockIdx.x + gx * blockIdx.y;
+  uint32_t l = b;
+  if (n % 8 == 0 && n >= 16) l = (b % 8) * (n / 8) + b / 8;
+  tx = l % gx;
+  ty = l / gx;
+}
+
 // FP32 accumulation in two levels: each K tile's products are summed in a
 // fresh accumulator (at most 16 MFMA roundings), which is then added to the
 // running sum.  A single chain over K = 256 rounds 128 times into a growing
@@ -94,7 +107,9 @@ __global__ __launch_bounds__(256) void k_gemm(const C* __restrict__ A, const C* 
   __shared__ int32_t offAk[KTAB_MAX], offBk[KTAB_MAX];
 
   const uint32_t Mt = g.M.total, Nt = g.N.total, Kt = g.K.total;
-  const uint32_t m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  uint32_t tile_x, tile_y;
+  xcd_tile(tile_x, tile_y);
+  const uint32_t m0 = tile_x * BM, n0 = tile_y * BN;
   const int tid = threadIdx.x;
   {
     int64_t oa, ob, oc;
@@ -237,7 +252,9 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
   __shared__ int32_t offAk[KT_MAX], offBk[KT_MAX];
 
   const uint32_t Mt = g.M.total, Nt = g.N.total, Kt = g.K.total;
-  const uint32_t m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  uint32_t tile_x, tile_y;
+  xcd_tile(tile_x, tile_y);
+  const uint32_t m0 = tile_x * BM, n0 = tile_y * BN;
   const int tid = threadIdx.x;
   {
     int64_t oa, ob, oc;
@@ -475,7 +492,9 @@ __global__ __launch_bounds__(256) void k_gemm_h(const c64* __restrict__ A, const
   __shared__ int32_t offAk[KT_MAX], offBk[KT_MAX];
 
   const uint32_t Mt = g.M.total, Nt = g.N.total, Kt = g.K.total;
-  const uint32_t m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  uint32_t tile_x, tile_y;
+  xcd_tile(tile_x, tile_y);
+  const uint32_t m0 = tile_x * BM, n0 = tile_y * BN;
   const int tid = threadIdx.x;
   {
     int64_t oa, ob, oc;
