@@ -5,7 +5,8 @@
 //
 // m, n, k are linear indices of the M / N / K index groups; element offsets
 // come from per-workgroup offset tables in LDS (row tables for the tile,
-// full K tables), so any stride pattern is read and written in place.
+// K tables in windows of 2048), so any stride pattern is read and written
+// in place.
 //
 // Complex product as four real MFMAs on split planes:
 //   Cr += Ar*Br + (-Ai)*Bi ;  Ci += Ar*Bi + Ai*Br
@@ -43,7 +44,8 @@ template <> struct mfma_t<double> {
 };
 
 constexpr int BK = 16;
-constexpr int KTAB_MAX = 2048;   // K tables held in LDS
+constexpr int KTAB_MAX = 2048;   // K offsets held in LDS at a time
+constexpr uint32_t GEMM_K_MAX = 1u << 24;
 
 // Workgroups are dealt to the 8 XCDs round-robin in launch order, and each
 // XCD has its own L2.  Renumber the (x, y) tiles of a batch entry so that
@@ -131,14 +133,6 @@ __global__ __launch_bounds__(256) void k_gemm(const C* __restrict__ A, const C* 
     offBn[t] = ob;
     offCn[t] = oc;
   }
-  for (uint32_t t = tid; t < Kt; t += 256) {
-    int32_t oa, ob;
-    group_offsets(g.K, t, oa, ob);
-    offAk[t] = oa;
-    offBk[t] = ob;
-  }
-  __syncthreads();
-
   const int wave = tid / 64, lane = tid % 64;
   const int wm = (wave / WGN) * WM, wn = (wave % WGN) * WN;
   const int li = lane % 16, lk = lane / 16;
@@ -146,57 +140,71 @@ __global__ __launch_bounds__(256) void k_gemm(const C* __restrict__ A, const C* 
   typename MF::acc accr[TM][TN], acci[TM][TN];
   zero_acc<MF, TM, TN>(accr, acci);
 
-  for (uint32_t k0 = 0; k0 < Kt; k0 += BK) {
-    typename MF::acc cr[TM][TN], ci[TM][TN];
-    if constexpr (CHUNKED) zero_acc<MF, TM, TN>(cr, ci);
-    // A tile BM x BK -> As[k][m]
-    for (int t = tid; t < BM * BK; t += 256) {
-      int m, k;
-      if (AK) { k = t % BK; m = t / BK; } else { m = t % BM; k = t / BM; }
-      C v;
-      v.x = 0; v.y = 0;
-      if (m0 + m < Mt && k0 + k < Kt) v = A[offAm[m] + offAk[k0 + k]];
-      As_r[k][m] = v.x;
-      As_i[k][m] = v.y;
-    }
-    // B tile BK x BN -> Bs[k][n]
-    for (int t = tid; t < BK * BN; t += 256) {
-      int n, k;
-      if (BK_FAST) { k = t % BK; n = t / BK; } else { n = t % BN; k = t / BN; }
-      C v;
-      v.x = 0; v.y = 0;
-      if (n0 + n < Nt && k0 + k < Kt) v = B[offBk[k0 + k] + offBn[n]];
-      Bs_r[k][n] = v.x;
-      Bs_i[k][n] = v.y;
+  // K runs in windows of KTAB_MAX indices, each with its own offset table
+  // (every K tile of a window ends with a barrier, so the next window's
+  // table may overwrite the last one's)
+  for (uint32_t kb = 0; kb < Kt; kb += KTAB_MAX) {
+    const uint32_t ke = Kt - kb < (uint32_t)KTAB_MAX ? Kt : kb + KTAB_MAX;
+    for (uint32_t t = tid; t < ke - kb; t += 256) {
+      int32_t oa, ob;
+      group_offsets(g.K, kb + t, oa, ob);
+      offAk[t] = oa;
+      offBk[t] = ob;
     }
     __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      R ar[TM], ai[TM], br[TN], bi[TN];
-#pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        ar[i] = As_r[kk + lk][wm + 16 * i + li];
-        ai[i] = As_i[kk + lk][wm + 16 * i + li];
+
+    for (uint32_t k0 = kb; k0 < ke; k0 += BK) {
+      typename MF::acc cr[TM][TN], ci[TM][TN];
+      if constexpr (CHUNKED) zero_acc<MF, TM, TN>(cr, ci);
+      // A tile BM x BK -> As[k][m]
+      for (int t = tid; t < BM * BK; t += 256) {
+        int m, k;
+        if (AK) { k = t % BK; m = t / BK; } else { m = t % BM; k = t / BM; }
+        C v;
+        v.x = 0; v.y = 0;
+        if (m0 + m < Mt && k0 + k < ke) v = A[offAm[m] + offAk[k0 - kb + k]];
+        As_r[k][m] = v.x;
+        As_i[k][m] = v.y;
       }
-#pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        br[j] = Bs_r[kk + lk][wn + 16 * j + li];
-        bi[j] = Bs_i[kk + lk][wn + 16 * j + li];
+      // B tile BK x BN -> Bs[k][n]
+      for (int t = tid; t < BK * BN; t += 256) {
+        int n, k;
+        if (BK_FAST) { k = t % BK; n = t / BK; } else { n = t % BN; k = t / BN; }
+        C v;
+        v.x = 0; v.y = 0;
+        if (n0 + n < Nt && k0 + k < ke) v = B[offBk[k0 - kb + k] + offBn[n]];
+        Bs_r[k][n] = v.x;
+        Bs_i[k][n] = v.y;
       }
+      __syncthreads();
 #pragma unroll
-      for (int i = 0; i < TM; ++i)
+      for (int kk = 0; kk < BK; kk += 4) {
+        R ar[TM], ai[TM], br[TN], bi[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+          ar[i] = As_r[kk + lk][wm + 16 * i + li];
+          ai[i] = As_i[kk + lk][wm + 16 * i + li];
+        }
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
-          typename MF::acc& xr = CHUNKED ? cr[i][j] : accr[i][j];
-          typename MF::acc& xi = CHUNKED ? ci[i][j] : acci[i][j];
-          xr = MF::op(ar[i], br[j], xr);
-          xr = MF::op(-ai[i], bi[j], xr);
-          xi = MF::op(ar[i], bi[j], xi);
-          xi = MF::op(ai[i], br[j], xi);
+          br[j] = Bs_r[kk + lk][wn + 16 * j + li];
+          bi[j] = Bs_i[kk + lk][wn + 16 * j + li];
         }
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            typename MF::acc& xr = CHUNKED ? cr[i][j] : accr[i][j];
+            typename MF::acc& xi = CHUNKED ? ci[i][j] : acci[i][j];
+            xr = MF::op(ar[i], br[j], xr);
+            xr = MF::op(-ai[i], bi[j], xr);
+            xi = MF::op(ar[i], bi[j], xi);
+            xi = MF::op(ai[i], br[j], xi);
+          }
+      }
+      if constexpr (CHUNKED) add_chunk<MF, TM, TN>(accr, acci, cr, ci);
+      __syncthreads();
     }
-    if constexpr (CHUNKED) add_chunk<MF, TM, TN>(accr, acci, cr, ci);
-    __syncthreads();
   }
 
   // Epilogue: accumulator register r of tile (i, j) holds D[MF::row(lk, r)][li].
@@ -275,12 +283,17 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
     offBn[t] = ob;
     offCn[t] = oc;
   }
+  // K tile offsets: a table when the tiles fit it, else decoded per tile
+  // (the same value in every lane)
   const uint32_t ntiles = Kt / BKT;
-  for (uint32_t t = tid; t < ntiles; t += 256) {
-    int32_t oa, ob;
-    group_offsets(g.K, t * BKT, oa, ob);
-    offAk[t] = oa;
-    offBk[t] = ob;
+  const bool ktab = ntiles <= (uint32_t)KT_MAX;
+  if (ktab) {
+    for (uint32_t t = tid; t < ntiles; t += 256) {
+      int32_t oa, ob;
+      group_offsets(g.K, t * BKT, oa, ob);
+      offAk[t] = oa;
+      offBk[t] = ob;
+    }
   }
   __syncthreads();
 
@@ -316,7 +329,13 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
   typedef typename std::conditional<sizeof(C) == 8, float4, double2>::type V16;
   V16 ra[ACH], rb[BCH];
   auto load_tile = [&](uint32_t kt) {
-    const int32_t ka = offAk[kt], kb = offBk[kt];
+    int32_t ka, kb;
+    if (ktab) {
+      ka = offAk[kt];
+      kb = offBk[kt];
+    } else {
+      group_offsets(g.K, kt * BKT, ka, kb);
+    }
 #pragma unroll
     for (int q = 0; q < ACH; ++q) ra[q] = *reinterpret_cast<const V16*>(A + ga[q] + ka);
 #pragma unroll
@@ -516,11 +535,14 @@ __global__ __launch_bounds__(256) void k_gemm_h(const c64* __restrict__ A, const
     offCn[t] = oc;
   }
   const uint32_t ntiles = Kt / BKT;
-  for (uint32_t t = tid; t < ntiles; t += 256) {
-    int32_t oa, ob;
-    group_offsets(g.K, t * BKT, oa, ob);
-    offAk[t] = oa;
-    offBk[t] = ob;
+  const bool ktab = ntiles <= (uint32_t)KT_MAX;
+  if (ktab) {
+    for (uint32_t t = tid; t < ntiles; t += 256) {
+      int32_t oa, ob;
+      group_offsets(g.K, t * BKT, oa, ob);
+      offAk[t] = oa;
+      offBk[t] = ob;
+    }
   }
   __syncthreads();
 
@@ -555,7 +577,13 @@ __global__ __launch_bounds__(256) void k_gemm_h(const c64* __restrict__ A, const
 
   float4 ra[ACH], rb[BCH];
   auto load_tile = [&](uint32_t kt) {
-    const int32_t ka = offAk[kt], kb = offBk[kt];
+    int32_t ka, kb;
+    if (ktab) {
+      ka = offAk[kt];
+      kb = offBk[kt];
+    } else {
+      group_offsets(g.K, kt * BKT, ka, kb);
+    }
 #pragma unroll
     for (int q = 0; q < ACH; ++q) ra[q] = *reinterpret_cast<const float4*>(A + ga[q] + ka);
 #pragma unroll
@@ -671,11 +699,11 @@ hipError_t launch_tile_h(const c64* A, const c64* B, c64* Out, const GemmArgs& g
 
 }  // namespace
 
-int gemm_k_limit() { return KTAB_MAX; }
+int gemm_k_limit() { return (int)GEMM_K_MAX; }
 
 template <typename C>
 hipError_t launch_gemm(const C* A, const C* B, C* Out, const GemmArgs& g, hipStream_t s) {
-  if (g.K.total > (uint32_t)KTAB_MAX || g.L.total < 1 || g.L.total > 65535) return hipErrorInvalidValue;
+  if (g.K.total > GEMM_K_MAX || g.L.total < 1 || g.L.total > 65535) return hipErrorInvalidValue;
   switch (g.tile) {
     case 0: return launch_tile<C, 32, 32, 2>(A, B, Out, g, s);
     case 1: return launch_tile<C, 64, 32, 2>(A, B, Out, g, s);
@@ -697,7 +725,7 @@ hipError_t launch_gemm(const C* A, const C* B, C* Out, const GemmArgs& g, hipStr
 }
 
 bool gemm16_applicable(const GemmArgs& g) {
-  return g.contig && g.K.nmodes > 0 && g.K.ext[0].d % 32 == 0 && g.K.total <= (uint32_t)KTAB_MAX &&
+  return g.contig && g.K.nmodes > 0 && g.K.ext[0].d % 32 == 0 && g.K.total <= GEMM_K_MAX &&
          g.L.total >= 1 && g.L.total <= 65535;
 }
 

@@ -6,11 +6,13 @@ calls raise, so nothing silently runs on the CPU.
 from __future__ import annotations
 
 import ctypes as ct
+import os
 import pathlib
 
 import numpy as np
 
-LIB_PATH = pathlib.Path(__file__).resolve().parent / "libpq_hip.so"
+# PQ_HIP_LIB names another build of the library (same C ABI).
+LIB_PATH = pathlib.Path(os.environ.get("PQ_HIP_LIB") or pathlib.Path(__file__).resolve().parent / "libpq_hip.so")
 
 PQ_OK = 0
 PQ_ERR_ARG = 1

@@ -10,7 +10,7 @@ from __future__ import annotations
 
 from . import _lib
 from .costmodel import merge
-from .executor import SMALL_ELEMS, assign_precision, lower_plan
+from .executor import MIXED_FP64_ELEMS, SMALL_ELEMS, assign_precision, lower_plan
 
 TILES = {_lib.PQ_C64: list(range(15)), _lib.PQ_C128: list(range(11)) + [12, 14]}
 
@@ -38,15 +38,16 @@ def tune_step(ctx, M, N, K, batch, dtype, reps=3, cache=None, split16=False):
 
 
 def autotune_plan(ctx, network, plan, sliced=(), group_bits=None, precision="mixed", reps=3,
-                  min_elems=SMALL_ELEMS, cache=None):
+                  min_elems=SMALL_ELEMS, cache=None, small_elems=MIXED_FP64_ELEMS):
     """Per contraction step of `plan`: (kc list, tile list, report).  Small
-    steps (grouped launches) keep the automatic choice."""
+    steps (grouped launches) keep the automatic choice; small_elems is the
+    plan's FP64 size limit (as in compile_plan)."""
     slots, steps, _ = lower_plan(network, plan, sliced)
     split16 = precision == "mixed16"
     if split16:
         precision = "mixed"
     code = _lib.PQ_C128 if (precision == "mixed" or ctx.code == _lib.PQ_C128) else _lib.PQ_C64
-    dts, steps, slots = assign_precision(slots, steps, code, precision)
+    dts, steps, slots = assign_precision(slots, steps, code, precision, small_elems=small_elems)
     n_sliced = len(list(sliced))
     gb = n_sliced if group_bits is None else group_bits
     cache = {} if cache is None else cache

@@ -52,7 +52,11 @@ def _arr(ctype, xs):
     return (ctype * max(1, len(xs)))(*xs)
 
 
-SMALL_ELEMS = 1 << 16   # steps up to this size run in FP64 under precision="mixed"
+SMALL_ELEMS = 1 << 16   # small intermediates: carried sliced bonds, partner re-layout
+# precision="mixed": steps whose operands and result have at most this many
+# elements run in FP64 (on the 6x6 depth-14 plan this halves the worst
+# amplitude error of the 2^16 limit at no measurable cost)
+MIXED_FP64_ELEMS = 1 << 20
 
 
 class SlotInfo:
@@ -219,7 +223,7 @@ def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf
 
 
 
-def assign_precision(slots, steps, ctx_code, precision, small_elems=SMALL_ELEMS, fp64_steps=()):
+def assign_precision(slots, steps, ctx_code, precision, small_elems=MIXED_FP64_ELEMS, fp64_steps=()):
     """Element type per slot and the step list with conversions:
     [(kind, a, b, c, M, N, K)], kind 0 = contract, 1 = convert a -> c.
 
@@ -348,12 +352,14 @@ def choose_group_bits(slots, n_sliced, budget_bytes, esize):
 
 
 def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=None, tiles=None,
-                 budget_bytes=64 << 30, precision="uniform", fp64_steps=(), carry_elems=SMALL_ELEMS):
+                 budget_bytes=64 << 30, precision="uniform", fp64_steps=(), carry_elems=SMALL_ELEMS,
+                 small_elems=MIXED_FP64_ELEMS):
     """Compile a pair plan of a network whose backend is a HIPBackend.
 
     precision: "uniform" (every step in the backend's type), "mixed" (FP64
     backend: small steps in FP64, large ones in complex64 on FP32 MFMA; see
-    assign_precision; fp64_steps forces contraction steps to FP64) or
+    assign_precision with small_elems; fp64_steps forces contraction steps
+    to FP64) or
     "mixed16" (as "mixed", with the complex64 GEMMs on FP16 hi/lo matrix
     cores where their shape allows).  kc / tiles, when given, are per contraction step of
     `plan` (kernel class and GEMM tile, -1 = automatic)."""
@@ -367,7 +373,7 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
     n_sliced = len(list(sliced))
     split16 = precision == "mixed16"
     dts, steps, slots = assign_precision(slots, steps, ctx.code, "mixed" if split16 else precision,
-                                         fp64_steps=fp64_steps)
+                                         small_elems=small_elems, fp64_steps=fp64_steps)
     if group_bits is None:
         esize = 8 if precision in ("mixed", "mixed16") or ctx.code == _lib.PQ_C64 else 16
         group_bits = choose_group_bits(slots, n_sliced, budget_bytes, esize)

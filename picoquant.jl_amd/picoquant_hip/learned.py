@@ -26,12 +26,22 @@ def _perturbed(order, rng, swaps, keep_last):
     return [q for q in o if q not in ks] + [q for q in o if q in ks]
 
 
-def candidate_plans(tn, rows, cols, open_q, n_candidates=64, seed=0, n_greedy=2):
-    """[(name, plan)]: n_candidates candidate pair plans for a grid network."""
+def candidate_plans(tn, rows, cols, open_q, n_candidates=64, seed=0, n_greedy=2, n_reconf=2):
+    """[(name, plan)]: n_candidates candidate pair plans for a grid network.
+    The column and row sweeps also enter after subtree reconfiguration
+    (planner.reconfigure, "-rc"), and so do the first n_reconf seeded
+    perturbations of them ("-rcp<k>", one window size)."""
     rng = np.random.default_rng(seed)
     base = {"col": PL.grid_site_order(rows, cols, "col", open_last=open_q),
             "row": PL.grid_site_order(rows, cols, "row", open_last=open_q)}
     cands = [(f"sweep-{k}", PL.sweep_plan(tn, v)) for k, v in base.items()]
+    for name, plan in list(cands):
+        cands.append((f"{name}-rc", PL.reconfigure(tn, plan)))
+    prng = np.random.default_rng(seed + 1)
+    for k in range(n_reconf):
+        name = "col" if k % 2 == 0 else "row"
+        plan = PL.sweep_plan(tn, _perturbed(base[name], prng, 2 + k, open_q))
+        cands.append((f"sweep-{name}-rcp{k}", PL.reconfigure(tn, plan, windows=(12,))))
     for k, v in base.items():
         rev = list(reversed([q for q in v if q not in set(open_q)])) + [q for q in v if q in set(open_q)]
         cands.append((f"sweep-{k}-rev", PL.sweep_plan(tn, rev)))

@@ -10,6 +10,10 @@
   sweep_plan     -- world lines first (each qubit's tensors contracted along
                     time into one site tensor), then the sites absorbed one by
                     one into a growing boundary in a given site order.
+  reconfigure    -- subtree reconfiguration: every window of up to W subtrees
+                    of a plan's contraction tree is re-ordered optimally
+                    (pq_netcon, the exact search behind netcon) when that
+                    lowers the MAC count.
   plan_ledger    -- exact per-step shapes, MACs and bytes of a plan.
 """
 from __future__ import annotations
@@ -214,6 +218,146 @@ def sweep_plan(network, site_order):
         counter += 1
         cur = f"node_{counter}"
     return plan
+
+
+# --------------------------------------------------------------------------- #
+# Subtree reconfiguration
+# --------------------------------------------------------------------------- #
+
+
+class _Tree:
+    """Contraction-tree node: a leaf (network node label) or a pair.  mask =
+    open indices (bitset over index ids); cost = MACs of the subtree."""
+
+    __slots__ = ("left", "right", "label", "mask", "cost")
+
+    def __init__(self, left=None, right=None, label=None, mask=0, sizer=None):
+        self.left, self.right, self.label = left, right, label
+        if label is not None:
+            self.mask, self.cost = mask, 0
+        else:
+            self.mask = left.mask ^ right.mask
+            self.cost = left.cost + right.cost + sizer(left.mask | right.mask)
+
+
+def _bits(mask):
+    out = []
+    while mask:
+        low = mask & -mask
+        out.append(low.bit_length() - 1)
+        mask ^= low
+    return out
+
+
+def reconfigure(network, plan, windows=(12, 16), rounds=12, max_work=1 << 24):
+    """A pair plan computing the same network with no more MACs.
+
+    The plan's contraction tree is walked from the root; at each pair node
+    the subtree is cut into a frontier of at most W subtrees (expanding the
+    one with the largest result first), the frontier's optimal contraction
+    order is found with pq_netcon, and it replaces the node's top of the
+    tree when it costs fewer MACs.  Passes repeat until one improves
+    nothing (at most `rounds` per window size), once per W in `windows`.
+    The MAC count is the reference's Metrics.flops for the plan.  Windows
+    whose exact search exceeds max_work pair evaluations are left as they
+    are."""
+    from .optimal_order import optimal_tree
+    from ._lib import PQError
+
+    ids, ext = {}, {}
+    leaves = {}
+    for lab, node in network.nodes.items():
+        m = 0
+        for x, d in zip(node.indices, node.dims):
+            if x not in ids:
+                ids[x] = len(ids)
+                ext[ids[x]] = int(d)
+            m |= 1 << ids[x]
+        leaves[lab] = m
+
+    def size(mask):
+        s = 1
+        for b in _bits(mask):
+            s *= ext[b]
+        return s
+
+    cur = {lab: _Tree(label=lab, mask=m) for lab, m in leaves.items()}
+    counter = network.counters["node"]
+    for a, b in plan:
+        t = _Tree(cur.pop(a), cur.pop(b), sizer=size)
+        counter += 1
+        cur[f"node_{counter}"] = t
+    if len(cur) != 1:
+        raise ValueError(f"plan leaves {len(cur)} tensors uncontracted")
+    (root,) = cur.values()
+
+    def window(t, W):
+        frontier = [t.left, t.right]
+        while len(frontier) < W:
+            inner = [i for i, f in enumerate(frontier) if f.label is None]
+            if not inner:
+                break
+            i = max(inner, key=lambda j: (size(frontier[j].mask), -j))
+            f = frontier.pop(i)
+            frontier += [f.left, f.right]
+        return frontier
+
+    def improve(t, W):
+        frontier = window(t, W)
+        old = t.cost - sum(f.cost for f in frontier)
+        idx = [_bits(f.mask) for f in frontier]
+        try:
+            tree, new = optimal_tree(idx, {b: ext[b] for bs in idx for b in bs}, max_work)
+        except PQError:
+            return t, False
+        if not new < old * (1 - 1e-12):
+            return t, False
+
+        def build(x):
+            if isinstance(x, int):
+                return frontier[x]
+            return _Tree(build(x[0]), build(x[1]), sizer=size)
+        return build(tree), True
+
+    def sweep(t, W):
+        changed = 0
+
+        # pre-order rebuild: improve a node, then its children
+        def rec(t):
+            nonlocal changed
+            if t.label is not None:
+                return t
+            t, ok = improve(t, W)
+            changed += ok
+            return _Tree(rec(t.left), rec(t.right), sizer=size)
+        return rec(t), changed
+
+    import sys
+    limit = sys.getrecursionlimit()
+    sys.setrecursionlimit(max(limit, 20 * len(leaves) + 1000))
+    try:
+        for W in windows:
+            for _ in range(rounds):
+                root, changed = sweep(root, W)
+                if not changed:
+                    break
+    finally:
+        sys.setrecursionlimit(limit)
+
+    out = []
+    counter = network.counters["node"]
+
+    def emit(t):
+        nonlocal counter
+        if t.label is not None:
+            return t.label
+        a = emit(t.left)
+        b = emit(t.right)
+        out.append([a, b])
+        counter += 1
+        return f"node_{counter}"
+    emit(root)
+    return out
 
 
 def grid_site_order(rows, cols, axis="col", open_last=()):

@@ -84,25 +84,31 @@ def slice_with_map(network, bond_labels, bond_values):
             lab = label_map[lab]
         return lab
 
+    views = set()
     for b, v in zip(bond_labels, bond_values):
         n1, n2 = network.edges[b].src, network.edges[b].dst
         for n in (n1, n2):
             view = f"node_{network.counters['node'] + 1}"
             replace_with_view(network, n, b, (v, v))
             label_map[n] = view
-    originals = {k: current(k) for k in label_map}
+            views.add(view)
+    # a node with several sliced bonds is replaced several times: map the
+    # original to the last view (the intermediate views are gone)
+    originals = {k: current(k) for k in label_map if k not in views}
     return originals
 
 
 def translate_plan(plan, label_map, old_counter, new_counter):
-    """Rename a pair plan for a network whose leaves were replaced (label_map)
-    and whose node counter moved from old_counter to new_counter."""
+    """Rename a pair plan for a network whose leaves were replaced (label_map:
+    original leaf -> its view) and whose node counter moved from old_counter
+    to new_counter.  Labels above old_counter are the plan's intermediates
+    and only shift."""
     shift = new_counter - old_counter
 
     def tr(lab):
-        if lab in label_map:
-            return label_map[lab]
         k = int(lab.rsplit("_", 1)[1])
-        return f"node_{k + shift}" if k > old_counter else lab
+        if k > old_counter:
+            return f"node_{k + shift}"
+        return label_map.get(lab, lab)
 
     return [[tr(a), tr(b)] for a, b in plan]

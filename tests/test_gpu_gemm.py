@@ -102,6 +102,35 @@ def test_boundary_times_site(ctx):
         ctx.free(x)
 
 
+@pytest.mark.parametrize("kfast", [True, False], ids=["contiguous-K", "strided-K"])
+def test_long_k_beyond_one_offset_table(ctx, kfast):
+    """K = 4608 > 2048: the any-stride kernel walks K in windows of offset
+    tables, the contiguous fast path decodes tile offsets on the fly; the
+    automatic dispatch takes the GEMM for this shape (rank-2 output 32 x 48,
+    the dep-chain shape of reconfigured grid plans)."""
+    rng = np.random.default_rng(21)
+    M, N, K = 32, 48, 4608
+    A = rand(rng, (K, M)) if kfast else rand(rng, (M, K))
+    ia = [1, -1] if kfast else [-1, 1]
+    B = rand(rng, (K, N))
+    ib = [1, -2]
+    check(ctx, A, ia, B, ib, gemm(ctx, A, ia, B, ib))
+    # split over two K modes with the slow one first in A (strided windows)
+    A2 = rand(rng, (72, M, 64))
+    B2 = rand(rng, (64, 72, N))
+    check(ctx, A2, [2, -1, 1], B2, [1, 2, -2], gemm(ctx, A2, [2, -1, 1], B2, [1, 2, -2]))
+    ha, hb = ctx.upload(A), ctx.upload(B)
+    ctx.profile_enable(True)
+    hc = ctx.contract(ha, ia, hb, ib)
+    got = ctx.download(hc)
+    st = ctx.profile_read(_lib.KC_GEMM)
+    ctx.profile_enable(False)
+    assert st["launches"] == 1
+    check(ctx, A, ia, B, ib, got)
+    for x in (ha, hb, hc):
+        ctx.free(x)
+
+
 GEMM16_TOL = 1e-5    # relative to sum |a||b|: hi/lo split drops lo*lo (~2^-22) and rounds lo
 
 
@@ -138,6 +167,32 @@ def test_gemm16_split_against_oracle(tile, orient):
         err = np.abs(got - want) / scale
         assert err.max() <= GEMM16_TOL, err.max()
         assert err.max() > 1e-9          # really ran in reduced precision
+    finally:
+        ctx.close()
+
+
+def test_gemm16_long_k():
+    """K = 4160 (130 tiles of 32, beyond the 64-entry tile table): tile
+    offsets decoded on the fly."""
+    ctx = _lib.Context(np.complex64)
+    try:
+        rng = np.random.default_rng(31)
+        M, N, K = 48, 40, 4160
+        A = rand(rng, (K, M)).astype(np.complex64)
+        B = rand(rng, (K, N)).astype(np.complex64)
+        ha, hb = ctx.upload(A), ctx.upload(B)
+        ctx.set_kernel_override(_lib.KC_GEMM16)
+        try:
+            hc = ctx.contract(ha, [1, -1], hb, [1, -2])
+        finally:
+            ctx.set_kernel_override(-1)
+        got = ctx.download(hc)
+        want = O.contract_tensors_data(A.astype(np.complex128), [1, -1], B.astype(np.complex128), [1, -2])
+        scale = O.contract_tensors_data(np.abs(A).astype(np.complex128), [1, -1],
+                                        np.abs(B).astype(np.complex128), [1, -2]).real
+        assert (np.abs(got - want) / scale).max() <= GEMM16_TOL
+        for h in (ha, hb, hc):
+            ctx.free(h)
     finally:
         ctx.close()
 
