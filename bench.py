@@ -226,6 +226,11 @@ def measured_traffic(kernel_class, d):
     t = json.loads(TRAFFIC_FILE.read_text())
     if t.get("kernel_class") != kernel_class:
         return None, None
+    # the summary holds for the plan it was measured on: same class bytes
+    prof_alg = t.get("class_alg_bytes_per_run")
+    if not prof_alg or abs(prof_alg - d["bytes"]) > 0.01 * d["bytes"]:
+        return None, {"stale": "profiles/" + TRAFFIC_FILE.name + " was measured on another plan",
+                      "alg_bytes_per_run": d["bytes"], "profiled_alg_bytes_per_run": prof_alg}
     per_launch = t["hbm_bytes_per_run"] / t["gemm_launches_per_run"]
     alg_per_launch = d["bytes"] / d["steps"]
     return per_launch, {"hbm_bytes_per_run": t["hbm_bytes_per_run"], "alg_bytes_per_run": d["bytes"],
@@ -319,16 +324,22 @@ def main():
     # Roofline of the dominant kernel class from an event-timed run.
     ms = p.profile()
     per = {}
-    for i in range(len(p.steps)):
+    for i, (kind, sa, sb, sc, M, N, K) in enumerate(p.steps):
         inf = p.step_info(i)
-        e = per.setdefault(inf["kc"], {"ms": 0.0, "flops": 0.0, "bytes": 0.0, "steps": 0})
+        e = per.setdefault(inf["kc"], {"ms": 0.0, "flops": 0.0, "bytes": 0.0, "steps": 0,
+                                       "flop_s": 0.0, "byte_s": 0.0})
         e["ms"] += ms[i]
         e["flops"] += inf["flops"]
         e["bytes"] += inf["bytes"]
         e["steps"] += 1
+        peak = PEAK["mfma_fp64"] if p.dtypes[sc] == _lib.PQ_C128 else PEAK["mfma_fp32"]
+        e["flop_s"] += inf["flops"] / peak
+        e["byte_s"] += inf["bytes"] / PEAK["hbm"]
     dom = max(per, key=lambda k: per[k]["ms"])
     d = per[dom]
-    if dom == _lib.KC_GEMM:
+    # the class is MFMA-bound when its steps' flop times outweigh their byte
+    # times at the peaks, else HBM-bound
+    if d["flop_s"] >= d["byte_s"]:
         achieved = d["flops"] / (d["ms"] * 1e-3)
         roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["mfma_fp32"] / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / PEAK["mfma_fp32"]}
@@ -341,6 +352,7 @@ def main():
     roof["launches_per_run"] = d["steps"]
     roof["class_ms_per_run"] = d["ms"]
     roof["classes"] = {_lib.KC_NAMES[k]: round(v["ms"], 3) for k, v in per.items()}
+    roof["class_bound_ms"] = {"flops": d["flop_s"] * 1e3, "bytes": d["byte_s"] * 1e3}
     # Whole-plan roofline: each step's lower bound max(flops / MFMA peak of
     # its element type, bytes / HBM peak), summed, against the measured run.
     lb = 0.0

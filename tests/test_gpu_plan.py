@@ -153,8 +153,8 @@ def test_grid5x5_d12_mixed_precision(golden_dir, nslice, group_bits):
     tn = grid_net(5, 5, 12, g["seed"], "".join(cfg), np.complex128)
     plan = PL.sweep_plan(tn, PL.grid_site_order(5, 5, open_last=g["open_qubits_1based"]))
     sliced = PL.choose_slices(tn, plan, nslice)[0] if nslice else []
-    # FP64 only up to 2^12 elements, so the sweep's large steps run in complex64
-    p = compile_plan(tn, plan, sliced, group_bits=group_bits, precision="mixed", small_elems=1 << 12)
+    # FP64 only up to 2^16 elements, so the sweep's large steps run in complex64
+    p = compile_plan(tn, plan, sliced, group_bits=group_bits, precision="mixed", small_elems=1 << 16)
     assert {d for d in p.dtypes} == {0, 1}                 # both element types in use
     assert any(k == 1 for k, *_ in p.steps)                # with conversions
     want = np.array([complex(a, b) for a, b in g["batch_amplitudes"]])
