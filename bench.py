@@ -266,6 +266,9 @@ def main():
                     help="reuse the autotuner's per-step choices from this file (written when missing), "
                          "so profiler runs trace only the plan's own kernels")
     ap.add_argument("--no-grid7", action="store_true", help="skip the 7x7 depth-12 precision comparison")
+    ap.add_argument("--no-fp64-check", action="store_true",
+                    help="skip the FP64 reference run (and the FP16-split variant that is checked against it); "
+                         "for profiler runs that should hold only the complex64 plan's kernels")
     ap.add_argument("--no-split16", action="store_true",
                     help="skip timing the FP16 hi/lo split variant of the benchmark plan")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -379,23 +382,25 @@ def main():
     log(f"[bench] planner: {planner_info}")
 
     # FP64 run of the same plan: the accuracy reference for the complex64 path.
-    tn64 = tn
-    p64 = compile_plan(tn64, plan, sliced, group_bits=N_SLICED)
-    p64.execute(True)
-    ref = p64.result().reshape(-1)
-    ctx64 = tn64.backend.ctx
-    ctx64.synchronize()
-    t2 = time.perf_counter()
-    p64.execute(True)
-    ctx64.synchronize()
-    t_fp64_batched = time.perf_counter() - t2
-    p64.close()
-    rel_err = float(np.max(np.abs(res64 - ref) / np.abs(ref)))
-    rel_err_norm = float(np.linalg.norm(res64 - ref) / np.linalg.norm(ref))
+    rel_err = rel_err_norm = t_fp64_batched = None
+    if not args.no_fp64_check:
+        p64 = compile_plan(tn, plan, sliced, group_bits=N_SLICED)
+        p64.execute(True)
+        ref = p64.result().reshape(-1)
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        p64.execute(True)
+        ctx.synchronize()
+        t_fp64_batched = time.perf_counter() - t2
+        p64.close()
+        rel_err = float(np.max(np.abs(res64 - ref) / np.abs(ref)))
+        rel_err_norm = float(np.linalg.norm(res64 - ref) / np.linalg.norm(ref))
+        log(f"[bench] max rel err vs FP64 {rel_err:.2e} (norm-wise {rel_err_norm:.2e}); "
+            f"batched FP64 run {t_fp64_batched:.3f}s")
 
     # The same plan with the complex64 GEMMs on FP16 hi/lo matrix cores.
     split16 = None
-    if not args.no_split16:
+    if not args.no_split16 and not args.no_fp64_check:
         ms16, res16, cls16 = run_variant(tn, plan, sliced, "mixed16", N_SLICED)
         split16 = {"ms_per_run": round(ms16, 3), "amps_per_s": 64e3 / ms16,
                    "rel_err_vs_fp64": float(np.max(np.abs(res16 - ref) / np.abs(ref))),
@@ -406,9 +411,6 @@ def main():
     if not args.no_grid7:
         grid7 = grid7x7_config()
         log(f"[bench] 7x7 d12: {grid7}")
-    log(f"[bench] max rel err vs FP64 {rel_err:.2e} (norm-wise {rel_err_norm:.2e}); "
-        f"batched FP64 run {t_fp64_batched:.3f}s")
-
     plain = None
     if not args.no_plain:
         t_slice, res_slice = plain_fp64_slice_time(plan, sliced, open_q)
@@ -444,7 +446,7 @@ def main():
         "rel_err_vs_fp64_normwise": rel_err_norm,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain": (amps / plain["amps_per_s"]) if plain else None,
-        "batched_fp64_amps_per_s": 64 / t_fp64_batched,
+        "batched_fp64_amps_per_s": 64 / t_fp64_batched if t_fp64_batched else None,
         "planner": planner_info,
         "fp16_split_variant": split16,
         "grid7x7_d12": grid7,

@@ -35,18 +35,38 @@ __global__ void k_fill_zero(double* p, int64_t n) {
     p[i] = 0.0;
 }
 
-// out[j] += sum_b C[b * stride + j] for b < nb (fixed order).
+// out[j] += sum_b C[b * stride + j] for b < nb, in FP64 and in a fixed
+// order (the same bits on every run).  A workgroup of 16 x 16 lanes takes 16
+// consecutive outputs: lane (tb, tj) sums entries b = tb, tb + 16, ... of
+// output j0 + tj (16 lanes of one b read 16 adjacent elements), then the 16
+// partial sums of each output are added pairwise in LDS.
 template <typename C>
-__global__ void k_accum(const C* __restrict__ src, int64_t elems, int64_t stride, int nb, double* __restrict__ out) {
-  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < elems; j += (int64_t)gridDim.x * blockDim.x) {
-    double re = out[2 * j], im = out[2 * j + 1];
-    for (int b = 0; b < nb; ++b) {
+__global__ __launch_bounds__(256) void k_accum(const C* __restrict__ src, int64_t elems, int64_t stride, int nb,
+                                               double* __restrict__ out) {
+  __shared__ double pr[16][17], pi[16][17];
+  const int tj = threadIdx.x % 16, tb = threadIdx.x / 16;
+  const int64_t j = (int64_t)blockIdx.x * 16 + tj;
+  double re = 0, im = 0;
+  if (j < elems) {
+    for (int b = tb; b < nb; b += 16) {
       C v = src[b * stride + j];
       re += (double)v.x;
       im += (double)v.y;
     }
-    out[2 * j] = re;
-    out[2 * j + 1] = im;
+  }
+  pr[tb][tj] = re;
+  pi[tb][tj] = im;
+  __syncthreads();
+  for (int h = 8; h > 0; h >>= 1) {
+    if (tb < h) {
+      pr[tb][tj] += pr[tb + h][tj];
+      pi[tb][tj] += pi[tb + h][tj];
+    }
+    __syncthreads();
+  }
+  if (tb == 0 && j < elems) {
+    out[2 * j] += pr[0][tj];
+    out[2 * j + 1] += pi[0][tj];
   }
 }
 
@@ -229,12 +249,12 @@ int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
       if (r) return r;
     }
     const void* src = slot_ptr(c, p, p->out_slot, grp);
-    unsigned ga = (unsigned)std::min<int64_t>((os.elems + 255) / 256, 1024);
+    const unsigned ga = (unsigned)((os.elems + 15) / 16);
     if (os.dtype == PQ_C128)
-      hipLaunchKernelGGL((k_accum<c128>), dim3(std::max(1u, ga)), dim3(256), 0, c->stream, (const c128*)src,
+      hipLaunchKernelGGL((k_accum<c128>), dim3(ga), dim3(256), 0, c->stream, (const c128*)src,
                          os.elems, os.elems, nbatch, p->out_dev);
     else
-      hipLaunchKernelGGL((k_accum<c64>), dim3(std::max(1u, ga)), dim3(256), 0, c->stream, (const c64*)src,
+      hipLaunchKernelGGL((k_accum<c64>), dim3(ga), dim3(256), 0, c->stream, (const c64*)src,
                          os.elems, os.elems, nbatch, p->out_dev);
     if (hipGetLastError() != hipSuccess) return fail(c, PQ_ERR_HIP, "accumulate kernel");
   }

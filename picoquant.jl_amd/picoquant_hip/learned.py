@@ -88,7 +88,12 @@ def select_plan(tn, rows, cols, open_q, n_sliced, model=None, n_candidates=64, s
         r["pred_sliced_us"] = model.plan_time(tn, r["plan"], sl) if model is not None else tot
     ok = [r for r in short if r.get("sliced") is not None and r["peak"] <= max_elems]
     if ok:
-        best = min(ok, key=lambda r: r["pred_sliced_us"])
+        # predictions within 2% of the best count as a tie, broken by the
+        # MAC count and then by candidate order (a model retrained on fresh
+        # timings then keeps picking the same plan)
+        top = min(r["pred_sliced_us"] for r in ok)
+        tied = [r for r in ok if r["pred_sliced_us"] <= top * 1.02]
+        best = min(tied, key=lambda r: r["sliced_macs"])
         flop_pick = min(ok, key=lambda r: r["sliced_macs"])
         fallback = False
     else:
