@@ -101,7 +101,8 @@ int pq_tensor_device_ptr(pq_ctx* ctx, pq_tensor t, void** base, int64_t* elem_of
 #define PQ_KC_BATCHED 5  /* batched small contractions (one group per problem) */
 #define PQ_KC_ACCUM 6    /* FP64 cross-slice accumulation */
 #define PQ_KC_GEMM16 7   /* complex64 GEMM on FP16 hi/lo matrix cores, FP32 accumulation */
-#define PQ_KC_COUNT 8
+#define PQ_KC_CHAIN 8    /* fused chain of small operators on one large tensor (plan steps of kind 2) */
+#define PQ_KC_COUNT 9
 int pq_profile_enable(pq_ctx* ctx, int enable);
 int pq_profile_read(pq_ctx* ctx, int kclass, int64_t* launches, double* total_ms,
                     double* alg_bytes, double* alg_flops);
@@ -169,10 +170,40 @@ typedef struct {
   const int32_t* slot_dtype;        /* [n_slots] PQ_C128 / PQ_C64 (NULL = context type);
                                        leaves must have the context's type */
   const int32_t* step_kind;         /* [n_steps] 0 = contract A x B -> C, 1 = convert A -> C
-                                       (element type change, same layout; NULL = all contract) */
+                                       (element type change, same layout), 2 = fused chain A -> C
+                                       (program in chain_words; B = its first operator slot);
+                                       NULL = all contract */
   const int32_t* step_arith;        /* [n_steps] complex64 GEMM arithmetic as for pq_set_arithmetic
                                        (NULL = 0, FP32 matrix cores) */
+  const int64_t* step_chain;        /* [n_steps] kind-2 steps: offset of the step's program in
+                                       chain_words (NULL when no step has kind 2) */
+  int64_t n_chain_words;
+  const int32_t* chain_words;       /* fused-chain programs (PQ_CHAIN_* below) */
 } pq_plan_desc;
+
+/* A fused chain applies L small complex64 operators one after the other to
+ * one large complex64 tensor, as L contraction steps would (each step sums
+ * an operator's K bits against bits of the running tensor and adds its M
+ * bits), without writing the L - 1 intermediates: one workgroup per
+ * configuration of the untouched ("spectator") modes gathers a tile of
+ * 2^TB elements into LDS, applies the operators on matrix cores in place,
+ * and scatters the tile to the output.  Every extent is split into bits
+ * (extent-2 factors); a bit has a tile position and strides.  Program words:
+ *   L, TB, NIN, NOUT, NSPEC,
+ *   NIN  x (tile position, stride in A)       gathered bits, A-stride order
+ *   NOUT x (tile position, stride in C)       scattered bits, C-stride order
+ *   NSPEC x (extent, stride in A, stride in C)
+ *   L x (operator slot, NK, NM, NR,
+ *        NK x (tile position, stride in the operator),
+ *        NM x (tile position, stride in the operator),
+ *        NR x tile position)
+ * An operator element is S[sum of k-bit strides + sum of m-bit strides]; the
+ * stage computes T[r, m] = sum_k S[m, k] T[r, k] for every r (bits at the R
+ * positions), writing m at the M positions.  The host checks every gather,
+ * scatter and operator access against its slot. */
+#define PQ_CHAIN_MAX_STAGES 8
+#define PQ_CHAIN_MAX_TILE_BITS 13
+#define PQ_CHAIN_MAX_OP_BITS 5
 
 int pq_plan_create(pq_ctx* ctx, const pq_plan_desc* desc, pq_plan** out);
 /* Enqueue one full run (all slices + FP64 accumulation); use_graph=1 captures

@@ -1,0 +1,155 @@
+// Fused chain of small operators on one large complex64 tensor.
+//
+// A run of pairwise steps (src/layer2.jl:346-405 applied along a plan) in
+// which each step contracts a small operand S_t (at most 2^5 x 2^5
+// elements) with the previous step's result rewrites a few bits of one big
+// tensor per step and is HBM-bound: every step reads and writes the whole
+// tensor.  Here the tensor's modes are split into bits; the bits any
+// operator touches form a tile of at most 2^13 elements.  One workgroup per
+// configuration of the other ("spectator") modes:
+//   1. gathers its tile from X into LDS (bits ordered by X stride, so
+//      adjacent lanes read adjacent addresses),
+//   2. applies the operators in order, in place:
+//        T[r, m] = sum_k S_t[m, k] T[r, k]
+//      as 16 x 16 x 4 complex blocks on v_mfma_f32_16x16x4_f32 (4 real
+//      MFMAs per block: Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br),
+//   3. scatters the tile to Y (bits ordered by Y stride).
+// Tile positions of every bit, per stage, come from the host program
+// (pq_hip.h, PQ_CHAIN_*); in place is safe because a wave reads all K
+// values of its 16 r's before it writes their M values, and different
+// waves own different r's (distinct R bits).
+#include "pq_internal.h"
+
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// LDS bank spreading: the tile is stored at swz(e), a bijection that is
+// linear over GF(2) (the low 5 bits are XORed with bits 5-9 and 10-14), so
+// swz(a | b) = swz(a) ^ swz(b) for addresses built from disjoint bit sets.
+// 16 lanes whose addresses differ by multiples of any power of two up to
+// 2^11 then fall on 16 distinct 8-byte bank slots.
+__device__ __forceinline__ int swz(int e) { return e ^ ((e >> 5) & 31) ^ ((e >> 10) & 31); }
+
+__device__ __forceinline__ floatx4 mfma(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Tile <-> tensor copy: element j = tid + 2^lo it; lane tid takes the low
+// tables' entry tid, iteration it the high tables' entry it.
+template <bool GATHER>
+__device__ __forceinline__ void move_tile(c64* tile, const c64* X, c64* Y, int32_t base, const int32_t* tab,
+                                          int lo, int hi, int lp, int lg, int hp, int hg, int tid) {
+  if (tid >= (1 << lo)) return;
+  const int p0 = swz(tab[lp + tid]);
+  const int32_t g0 = base + tab[lg + tid];
+  const int nit = 1 << hi;
+  for (int it = 0; it < nit; ++it) {
+    const int p = p0 ^ swz(tab[hp + it]);
+    const int32_t g = g0 + tab[hg + it];
+    if (GATHER)
+      tile[p] = X[g];
+    else
+      Y[g] = tile[p];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_chain(const ChainArgs a) {
+  extern __shared__ c64 tile[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int li = lane & 15, lg = lane >> 4;
+  const int32_t* tab = a.tab;
+  int32_t ox, oy;
+  group_offsets(a.spec, blockIdx.x, ox, oy);
+
+  move_tile<true>(tile, a.X, nullptr, ox, tab, a.ilo, a.ihi, a.ilp, a.ilg, a.ihp, a.ihg, tid);
+  __syncthreads();
+
+  for (int s = 0; s < a.nst; ++s) {
+    const ChainStage& g = a.st[s];
+    const int nk = 1 << g.nk, nm = 1 << g.nm;
+    const int nkc = (nk + 3) / 4, nmb = (nm + 15) / 16;
+    // This lane's operator values (MFMA A operand: row m = 16 mb + li,
+    // column k = 4 kc + lg), zero outside the operator; its K and M tile
+    // offsets (B operand row k = 4 kc + lg; output row m = 16 mb + 4 lg + i);
+    // the tile offset of its column li within a block of 16 r's.
+    c64 sv[8][2];
+    float svn[8][2];   // -Im of the operator values
+    int kof[8], mof[2][4];
+#pragma unroll
+    for (int kc = 0; kc < 8; ++kc) {
+      const int k = 4 * kc + lg;
+      const bool kv = kc < nkc && k < nk;
+      kof[kc] = kv ? swz(tab[g.kp + k]) : 0;
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb) {
+        const int m = 16 * mb + li;
+        c64 v;
+        v.x = 0.f;
+        v.y = 0.f;
+        if (kv && mb < nmb && m < nm) v = g.S[tab[g.sk + k] + tab[g.sm + m]];
+        sv[kc][mb] = v;
+        svn[kc][mb] = -v.y;
+      }
+    }
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int m = 16 * mb + 4 * lg + i;
+        mof[mb][i] = (mb < nmb && m < nm) ? swz(tab[g.mp + m]) : 0;
+      }
+    const bool rv = li < (1 << g.rlo);
+    const int rlo = swz(tab[g.rl + (rv ? li : 0)]);
+    const int nrb = 1 << g.rhi;
+    for (int rb = wave; rb < nrb; rb += 4) {
+      const int roff = rlo ^ swz(tab[g.rh + rb]);
+      // every K value of these r's is read before any M value is written
+      c64 bv[8];
+#pragma unroll
+      for (int kc = 0; kc < 8; ++kc) {
+        c64 v;
+        v.x = 0.f;
+        v.y = 0.f;
+        if (kc < nkc && 4 * kc + lg < nk) v = tile[roff ^ kof[kc]];
+        bv[kc] = v;
+      }
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb) {
+        if (mb >= nmb) break;
+        floatx4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < 8; ++kc) {
+          if (kc >= nkc) break;
+          cr = mfma(sv[kc][mb].x, bv[kc].x, cr);
+          ci = mfma(sv[kc][mb].x, bv[kc].y, ci);
+          cr = mfma(svn[kc][mb], bv[kc].y, cr);
+          ci = mfma(sv[kc][mb].y, bv[kc].x, ci);
+        }
+        // accumulator register i holds row 4 lg + i, column li
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (rv && 16 * mb + 4 * lg + i < nm) {
+            c64 v;
+            v.x = cr[i];
+            v.y = ci[i];
+            tile[roff ^ mof[mb][i]] = v;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  move_tile<false>(tile, nullptr, a.Y, oy, tab, a.olo, a.ohi, a.olp, a.olg, a.ohp, a.ohg, tid);
+}
+
+}  // namespace
+
+hipError_t launch_chain(const ChainArgs& a, hipStream_t s) {
+  if (a.tb < 0 || a.tb > CH_TB || a.nst < 1 || a.nst > CH_ST || a.spec.total < 1) return hipErrorInvalidValue;
+  const size_t lds = sizeof(c64) << a.tb;
+  hipLaunchKernelGGL(k_chain, dim3(a.spec.total), dim3(256), lds, s, a);
+  return hipGetLastError();
+}

@@ -78,3 +78,33 @@ hipError_t launch_gemm16(const c64* A, const c64* B, c64* Out, const GemmArgs& a
 
 // Number of partial blocks the split-K reduction uses for a given K extent.
 int reduce_blocks_for(uint32_t ktotal);
+
+// Fused chain of small complex64 operators on one large tensor (plan steps
+// of kind 2; the program format is documented with PQ_CHAIN_* in pq_hip.h).
+// The host expands each program into int32 address tables (logical tile
+// addresses and element offsets) in one device buffer; a bit list of n bits
+// is split into a low part (up to 8 bits: gather / scatter, 4 bits: R) taken
+// from the lane and a high part taken from the loop counter, each with its
+// own table, so the kernel only adds table entries.
+#define CH_TB 13   // PQ_CHAIN_MAX_TILE_BITS
+#define CH_OB 5    // PQ_CHAIN_MAX_OP_BITS
+#define CH_ST 8    // PQ_CHAIN_MAX_STAGES
+struct ChainStage {
+  const c64* S;
+  int32_t nk, nm;          // 2^nk operator columns, 2^nm rows
+  int32_t rlo, rhi;        // R bits from the lane (<= 4) and from the block counter
+  int32_t kp, mp, sk, sm;  // table offsets: K / M tile addresses, K / M operator offsets
+  int32_t rl, rh;          // table offsets: R tile addresses (low, high part)
+};
+struct ChainArgs {
+  const c64* X;
+  c64* Y;
+  const int32_t* tab;
+  int32_t tb, nst;
+  int32_t ilo, ihi, olo, ohi;              // gather / scatter bits from the lane, from the counter
+  int32_t ilp, ilg, ihp, ihg;              // gather tables: low / high tile address, element offset
+  int32_t olp, olg, ohp, ohg;              // scatter tables
+  Group spec;   // spectator modes: s0 = stride in X, s1 = stride in Y
+  ChainStage st[CH_ST];
+};
+hipError_t launch_chain(const ChainArgs& a, hipStream_t s);

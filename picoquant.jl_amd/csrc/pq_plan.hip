@@ -99,8 +99,10 @@ struct Slot {
 };
 
 struct PStep {
-  int kind = 0;            // 0 contract, 1 convert
+  int kind = 0;            // 0 contract, 1 convert, 2 fused chain
   int a, b, c;
+  std::vector<int> ops;    // fused chain: operator slot of each stage
+  ChainArgs chain;         // fused chain: program (pointers set at launch)
   bool dep;                // depends on a sliced bond (batched / selects per slice)
   bool phase2 = false;     // depends on a bond above the slice group: rerun per group
   StepSpec spec;           // resolved; batch strides for phase 2 steps
@@ -127,6 +129,7 @@ struct pq_plan {
   std::vector<int> order1, order2;   // step indices of phase 1 / phase 2
   std::vector<Launch> seq1, seq2;    // launch sequences of phase 1 / phase 2
   MultiDesc* descs = nullptr;        // device copies of the grouped descriptors
+  int32_t* chain_tab = nullptr;      // device copy of the fused-chain address tables
   int n_sliced = 0, group_bits = 0;
   int64_t n_groups = 1;
   int out_slot = -1;
@@ -171,7 +174,156 @@ void free_plan(pq_plan* p) {
   if (p->scratch) hipFree(p->scratch);
   if (p->out_dev) hipFree(p->out_dev);
   if (p->descs) hipFree(p->descs);
+  if (p->chain_tab) hipFree(p->chain_tab);
   delete p;
+}
+
+// Whether `span` (largest element offset read or written) stays inside slot s.
+bool slot_holds(pq_ctx* c, const Slot& sl, int64_t span) {
+  if (!sl.leaf) return span < sl.elems;
+  Tensor* t = get(c, sl.leaf);
+  int64_t full = 0;
+  for (int q = 0; q < t->rank; ++q) full += (t->dims[q] - 1) * t->strides[q];
+  return span <= full;
+}
+
+// Appends the address table of a bit list (bits [b0, b1), positions and
+// strides) to tab: entry t is the sum over the set bits of t of
+// 2^position (pos = true) or of the stride.  Returns the table's offset.
+int32_t bit_table(std::vector<int32_t>& tab, const std::vector<int32_t>& vals, int b0, int b1, bool pos) {
+  const int32_t off = (int32_t)tab.size();
+  for (int64_t t = 0; t < ((int64_t)1 << (b1 - b0)); ++t) {
+    int64_t v = 0;
+    for (int b = b0; b < b1; ++b)
+      if ((t >> (b - b0)) & 1) v += pos ? ((int64_t)1 << vals[b]) : (int64_t)vals[b];
+    tab.push_back((int32_t)v);
+  }
+  return off;
+}
+
+// Reads and checks the program of fused-chain step i (format: PQ_CHAIN_* in
+// pq_hip.h) into st.chain / st.ops and its address tables into tab.
+// Returns an error message or nullptr.
+const char* parse_chain(pq_ctx* c, pq_plan* p, const pq_plan_desc* d, int i, PStep& st,
+                        const std::vector<int>& produced, std::vector<int32_t>& tab) {
+  if (!d->step_chain || !d->chain_words) return "fused chain without a program";
+  int64_t q = d->step_chain[i];
+  bool ok = true;
+  auto w = [&]() -> int32_t {
+    if (q < 0 || q >= d->n_chain_words) {
+      ok = false;
+      return 0;
+    }
+    return d->chain_words[q++];
+  };
+  ChainArgs& a = st.chain;
+  memset(&a, 0, sizeof(a));
+  const int L = w(), tb = w(), nin = w(), nout = w(), nspec = w();
+  if (!ok || L < 1 || L > CH_ST || tb < 0 || tb > CH_TB || nin < 0 || nin > tb || nout < 0 || nout > tb ||
+      nspec < 0 || nspec > PQ_MAX_MODES)
+    return "bad fused-chain header";
+  a.tb = tb;
+  a.nst = L;
+  const Slot &X = p->slots[st.a], &Y = p->slots[st.c];
+  // bit lists: tile position and stride per bit; no position twice
+  auto bits = [&](int n, std::vector<int32_t>& pos, std::vector<int32_t>* str, int64_t& span,
+                  uint32_t& used) -> bool {
+    pos.resize(n);
+    if (str) str->resize(n);
+    for (int b = 0; b < n; ++b) {
+      pos[b] = w();
+      const int32_t sv = str ? w() : 0;
+      if (!ok || pos[b] < 0 || pos[b] >= tb || sv < 0 || ((used >> pos[b]) & 1)) return false;
+      used |= 1u << pos[b];
+      if (str) (*str)[b] = sv;
+      span += sv;
+    }
+    return true;
+  };
+  std::vector<int32_t> ipos, istr, opos, ostr;
+  int64_t xspan = 0, yspan = 0, unused = 0;
+  uint32_t used = 0;
+  if (!bits(nin, ipos, &istr, xspan, used)) return "bad gathered bit";
+  used = 0;
+  if (!bits(nout, opos, &ostr, yspan, used)) return "bad scattered bit";
+  std::vector<Mode> spec;
+  for (int m = 0; m < nspec; ++m) {
+    Mode md;
+    md.ext = w();
+    md.s0 = w();
+    md.s1 = w();
+    if (md.ext < 1 || md.s0 < 0 || md.s1 < 0) return "bad spectator mode";
+    xspan += (md.ext - 1) * md.s0;
+    yspan += (md.ext - 1) * md.s1;
+    spec.push_back(md);
+  }
+  if (!ok || !fill_group(a.spec, spec)) return "bad spectator modes";
+  if (X.dtype != PQ_C64 || Y.dtype != PQ_C64) return "fused chain needs complex64 tensors";
+  if (X.mask || Y.mask) return "fused chain on a slice-dependent tensor";
+  for (int64_t v : X.slice_stride)
+    if (v) return "fused chain input keeps a sliced bond";
+  for (int64_t v : Y.slice_stride)
+    if (v) return "fused chain output keeps a sliced bond";
+  if (!slot_holds(c, X, xspan)) return "gather out of range";
+  if (!slot_holds(c, Y, yspan)) return "scatter out of range";
+  if (((int64_t)a.spec.total << nout) != Y.elems) return "scatter does not cover the output";
+  a.ilo = std::min(nin, 8);
+  a.ihi = nin - a.ilo;
+  a.olo = std::min(nout, 8);
+  a.ohi = nout - a.olo;
+  a.ilp = bit_table(tab, ipos, 0, a.ilo, true);
+  a.ilg = bit_table(tab, istr, 0, a.ilo, false);
+  a.ihp = bit_table(tab, ipos, a.ilo, nin, true);
+  a.ihg = bit_table(tab, istr, a.ilo, nin, false);
+  a.olp = bit_table(tab, opos, 0, a.olo, true);
+  a.olg = bit_table(tab, ostr, 0, a.olo, false);
+  a.ohp = bit_table(tab, opos, a.olo, nout, true);
+  a.ohg = bit_table(tab, ostr, a.olo, nout, false);
+  st.ops.clear();
+  for (int t = 0; t < L; ++t) {
+    ChainStage& g = a.st[t];
+    const int os = w();
+    const int nk = w(), nm = w(), nr = w();
+    if (!ok || os < 0 || os >= (int)p->slots.size() || nk < 0 || nk > CH_OB || nm < 0 || nm > CH_OB || nr < 0 ||
+        nr > CH_TB)
+      return "bad fused-chain stage";
+    const Slot& S = p->slots[os];
+    if (!S.leaf && produced[os] < 0) return "operator read before it is produced";
+    if (S.dtype != PQ_C64 || S.mask) return "fused-chain operator must be an unbatched complex64 tensor";
+    for (int64_t v : S.slice_stride)
+      if (v) return "fused-chain operator keeps a sliced bond";
+    std::vector<int32_t> kpos, kstr, mpos, mstr, rpos;
+    int64_t sspan = 0;
+    uint32_t kr = 0, mr = 0, rr = 0;
+    if (!bits(nk, kpos, &kstr, sspan, kr)) return "bad K bit";
+    if (!bits(nm, mpos, &mstr, sspan, mr)) return "bad M bit";
+    if (!bits(nr, rpos, nullptr, unused, rr)) return "bad R bit";
+    // R must be disjoint from K and from M: every (r, k) and (r, m) address
+    // is then distinct across r, which the in-place update relies on
+    if ((rr & kr) || (rr & mr)) return "R bits overlap K or M bits";
+    if (!slot_holds(c, S, sspan)) return "operator access out of range";
+    g.nk = nk;
+    g.nm = nm;
+    g.rlo = std::min(nr, 4);
+    g.rhi = nr - g.rlo;
+    g.kp = bit_table(tab, kpos, 0, nk, true);
+    g.sk = bit_table(tab, kstr, 0, nk, false);
+    g.mp = bit_table(tab, mpos, 0, nm, true);
+    g.sm = bit_table(tab, mstr, 0, nm, false);
+    g.rl = bit_table(tab, rpos, 0, g.rlo, true);
+    g.rh = bit_table(tab, rpos, g.rlo, nr, true);
+    st.ops.push_back(os);
+    st.level = std::max(st.level, 1 + (S.leaf ? 0 : p->steps[S.producer].level));
+    p->slots[os].last_use = i;
+    st.flops += 8.0 * (double)((int64_t)1 << (nr + nk + nm)) * (double)a.spec.total;
+    st.bytes += 8.0 * (double)S.elems;
+  }
+  st.bytes += 8.0 * (double)a.spec.total * (double)(((int64_t)1 << nin) + ((int64_t)1 << nout));
+  st.spec.kc = PQ_KC_CHAIN;
+  st.spec.tile = -1;
+  st.dep = false;
+  st.phase2 = false;
+  return nullptr;
 }
 
 int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
@@ -206,6 +358,15 @@ int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
           hipLaunchKernelGGL((k_convert<c64, c64>), dim3(std::max(1u, gc)), dim3(256), 0, c->stream,
                              (const c64*)src, (c64*)dst, n);
         if (hipGetLastError() != hipSuccess) r = fail(c, PQ_ERR_HIP, "convert kernel");
+      } else if (st.kind == 2) {
+        ChainArgs a = st.chain;
+        a.tab = p->chain_tab;
+        a.X = (const c64*)slot_ptr(c, p, st.a, group);
+        a.Y = (c64*)slot_ptr(c, p, st.c, group);
+        for (int t = 0; t < a.nst; ++t) a.st[t].S = (const c64*)slot_ptr(c, p, st.ops[t], group);
+        ProfScope ps(c, PQ_KC_CHAIN, st.bytes, st.flops);
+        hipError_t e = launch_chain(a, c->stream);
+        if (e != hipSuccess) r = hipfail(c, e, "fused chain kernel");
       } else {
         r = launch_step(c, slot_ptr(c, p, st.a, group), slot_ptr(c, p, st.b, group), slot_ptr(c, p, st.c, group),
                         st.spec, p->scratch);
@@ -316,12 +477,13 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
   }
   // Steps.
   int64_t mpos = 0;
+  std::vector<int32_t> chain_tab;   // fused-chain address tables (uploaded below)
   std::vector<int> produced(d->n_slots, -1);
   p->steps.resize(d->n_steps);
   for (int i = 0; i < d->n_steps; ++i) {
     PStep& st = p->steps[i];
     st.kind = d->step_kind ? d->step_kind[i] : 0;
-    if (st.kind != 0 && st.kind != 1) return bad("bad step kind");
+    if (st.kind < 0 || st.kind > 2) return bad("bad step kind");
     st.a = d->step_slots[3 * i];
     st.b = d->step_slots[3 * i + 1];
     st.c = d->step_slots[3 * i + 2];
@@ -338,6 +500,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     p->slots[st.a].last_use = i;
     p->slots[st.b].last_use = i;
     const int nm = d->step_nmodes[3 * i], nn = d->step_nmodes[3 * i + 1], nk = d->step_nmodes[3 * i + 2];
+    if (st.kind == 2 && (nm || nn || nk)) return bad("a fused chain has no modes of its own");
     auto take = [&](int n) {
       std::vector<Mode> v;
       for (int k = 0; k < n; ++k, ++mpos) v.push_back({d->step_modes[3 * mpos], d->step_modes[3 * mpos + 1],
@@ -354,6 +517,14 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     st.dep = Cs.batched;
     st.phase2 = (Cs.mask >> d->group_bits) != 0;
     if ((A.mask | B.mask) & ~Cs.mask) return bad("result independent of a sliced bond its operand depends on");
+    if (st.kind == 2) {
+      const char* why = parse_chain(c, p, d, i, st, produced, chain_tab);
+      if (why) return bad(std::string("step ") + std::to_string(i) + ": " + why);
+      p->alg_flops += st.flops;
+      p->alg_bytes += st.bytes;
+      p->order1.push_back(i);
+      continue;
+    }
     if (st.kind == 1) {
       if (A.leaf || A.elems != Cs.elems || A.mask != Cs.mask || A.slice_stride != Cs.slice_stride)
         return bad("convert needs a same-shaped intermediate");
@@ -605,6 +776,15 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
       (void)hipGetLastError();
       free_plan(p);
       return fail(c, PQ_ERR_OOM, "descriptor upload failed");
+    }
+  }
+  if (!chain_tab.empty()) {
+    if (hipMalloc((void**)&p->chain_tab, chain_tab.size() * sizeof(int32_t)) != hipSuccess ||
+        hipMemcpy(p->chain_tab, chain_tab.data(), chain_tab.size() * sizeof(int32_t), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+      (void)hipGetLastError();
+      free_plan(p);
+      return fail(c, PQ_ERR_OOM, "fused-chain table upload failed");
     }
   }
   p->launches = 1 + (int64_t)p->seq1.size() + p->n_groups * ((int64_t)p->seq2.size() + 1);

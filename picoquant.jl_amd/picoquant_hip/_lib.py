@@ -26,9 +26,13 @@ PQ_C64 = 1
 PQ_MAX_RANK = 64
 PQ_GEMM_TILES = 15
 
-KC_STREAM, KC_GEMM, KC_GENERIC, KC_REDUCE, KC_COPY, KC_BATCHED, KC_ACCUM, KC_GEMM16 = range(8)
+KC_STREAM, KC_GEMM, KC_GENERIC, KC_REDUCE, KC_COPY, KC_BATCHED, KC_ACCUM, KC_GEMM16, KC_CHAIN = range(9)
 KC_NAMES = {KC_STREAM: "stream", KC_GEMM: "gemm", KC_GENERIC: "generic", KC_REDUCE: "reduce",
-            KC_COPY: "copy", KC_BATCHED: "batched", KC_ACCUM: "accum", KC_GEMM16: "gemm16"}
+            KC_COPY: "copy", KC_BATCHED: "batched", KC_ACCUM: "accum", KC_GEMM16: "gemm16",
+            KC_CHAIN: "chain"}
+PQ_CHAIN_MAX_STAGES = 8
+PQ_CHAIN_MAX_TILE_BITS = 13
+PQ_CHAIN_MAX_OP_BITS = 5
 
 # Every exported symbol, with (restype, argtypes).  tests/test_boundary.py
 # checks this list against include/pq_hip.h.

@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import ctypes as ct
 import math
+from collections import defaultdict
 
 import numpy as np
 
@@ -45,6 +46,9 @@ class PlanDesc(ct.Structure):
         ("slot_dtype", ct.POINTER(ct.c_int32)),
         ("step_kind", ct.POINTER(ct.c_int32)),
         ("step_arith", ct.POINTER(ct.c_int32)),
+        ("step_chain", ct.POINTER(ct.c_int64)),
+        ("n_chain_words", ct.c_int64),
+        ("chain_words", ct.POINTER(ct.c_int32)),
     ]
 
 
@@ -267,6 +271,213 @@ def assign_precision(slots, steps, ctx_code, precision, small_elems=MIXED_FP64_E
     return dt, out, slots
 
 
+# --------------------------------------------------------------------------- #
+# Fused chains: runs of steps that each apply a small operator to the
+# previous step's (large) result, executed as one kernel (PQ_KC_CHAIN)
+# --------------------------------------------------------------------------- #
+
+CHAIN_TILE_BITS = 13     # tile of 2^13 complex64 = 64 KiB of LDS per workgroup (fewer, longer chains
+                         # beat 2^12 tiles with more workgroups per CU on the 6x6 d14 plan)
+CHAIN_MIN_ELEMS = 1 << 16
+CHAIN_OP_ELEMS = 1024
+
+
+def _bits(slot, labels):
+    """(label, j) bits of labels in a slot, j-th factor 2 of the extent, with
+    their strides; None when an extent is not a power of two."""
+    out = []
+    for x in labels:
+        e = slot.ext[x]
+        if e & (e - 1):
+            return None
+        for j in range(e.bit_length() - 1):
+            out.append(((x, j), slot.stride[x] << j))
+    return out
+
+
+def chain_program(slots, steps, group, tile_bits=CHAIN_TILE_BITS):
+    """Program words (include/pq_hip.h, PQ_CHAIN_*) for the contraction steps
+    `group` (indices into steps, each applying its small operand to the
+    previous one's result), or None when a limit is exceeded."""
+    first = steps[group[0]]
+    xs = [first[1], first[2]]
+    X0 = max(xs, key=lambda s: slots[s].elems)
+    XL = steps[group[-1]][3]
+    stages = []
+    touched = set()
+    x = X0
+    for g in group:
+        kind, a, b, c = steps[g][:4]
+        big, op = (a, b) if a == x else (b, a)
+        if big != x:
+            return None
+        K = [l for l in slots[op].idx if l in set(slots[big].idx)]
+        M = [l for l in slots[op].idx if l not in set(slots[big].idx)]
+        stages.append((op, K, M))
+        touched |= set(K) | set(M)
+        x = c
+    X, Y = slots[X0], slots[XL]
+    spect = [l for l in X.idx if l not in touched]
+    if sorted(spect) != sorted(l for l in Y.idx if l not in touched):
+        return None
+    tile0 = _bits(X, [l for l in X.idx if l in touched])
+    if tile0 is None or _bits(Y, Y.idx) is None or _bits(X, X.idx) is None:
+        return None
+    if any(_bits(slots[op], slots[op].idx) is None for op, _, _ in stages):
+        return None
+
+    # tile size over the stages for a given set of passenger labels
+    def size_with(pass_labels):
+        n = len(tile0) + sum(len(_bits(X, [l])) for l in pass_labels)
+        peak = n
+        for op, K, M in stages:
+            n += len(_bits(slots[op], M)) - len(_bits(slots[op], K))
+            peak = max(peak, n)
+        return peak
+
+    if size_with([]) > tile_bits:
+        return None
+    # passengers: spectators with the smallest output (then input) strides,
+    # so that the tile's stores (and loads) run over >= 16 adjacent elements
+    passengers = []
+    for ref in (Y, X):
+        low = sorted(ref.idx, key=lambda l: ref.stride[l])
+        run = 0
+        for l in low:
+            if run >= 4:
+                break
+            if l in spect and l not in passengers:
+                if size_with(passengers + [l]) > tile_bits:
+                    break
+                passengers.append(l)
+            run += len(_bits(ref, [l]))
+    spect = [l for l in spect if l not in passengers]
+    # stage 0: tile bits in input-stride order take positions 0, 1, ...
+    gather = sorted(_bits(X, [l for l in X.idx if l in touched or l in passengers]), key=lambda t: t[1])
+    pos = {bit: i for i, (bit, _) in enumerate(gather)}
+    free = []
+    top = len(gather)
+    words_st = []
+    for op, K, M in stages:
+        S = slots[op]
+        kb, mb = _bits(S, K), _bits(S, M)
+        if kb is None or mb is None or len(kb) > _lib.PQ_CHAIN_MAX_OP_BITS or len(mb) > _lib.PQ_CHAIN_MAX_OP_BITS:
+            return None
+        kpos = [(pos.pop(bit), s) for bit, s in kb]
+        free = sorted(free + [p for p, _ in kpos])
+        rbits = sorted(pos.values())
+        mpos = []
+        for bit, s in mb:
+            if free:
+                p = free.pop(0)
+            else:
+                p = top
+                top += 1
+            pos[bit] = p
+            mpos.append((p, s))
+        words_st.append((op, kpos, mpos, rbits))
+    if top > tile_bits:
+        return None
+    final = {bit[0] for bit in pos}
+    scatter = sorted(((pos[bit], s) for bit, s in _bits(Y, [l for l in Y.idx if l in final])), key=lambda t: t[1])
+    if len(scatter) != len(pos):
+        return None
+    words = [len(stages), top, len(gather), len(scatter), len(spect)]
+    for i, (bit, s) in enumerate(gather):
+        words += [i, s]
+    for p, s in scatter:
+        words += [p, s]
+    for l in spect:
+        words += [X.ext[l], X.stride[l], Y.stride[l]]
+    for op, kpos, mpos, rbits in words_st:
+        words += [op, len(kpos), len(mpos), len(rbits)]
+        for p, s in kpos + mpos:
+            words += [p, s]
+        words += rbits
+    return words
+
+
+def fuse_chains(slots, steps, dts, tile_bits=CHAIN_TILE_BITS, min_elems=CHAIN_MIN_ELEMS,
+                op_elems=CHAIN_OP_ELEMS):
+    """Replace runs of complex64 steps that each contract a small operand
+    (<= op_elems) with the previous step's large (>= min_elems) result by
+    fused-chain steps (kind 2), as many consecutive steps per fused step as
+    fit a tile of 2^tile_bits.  Only unbatched steps that keep no sliced bond
+    take part.  Returns (steps, programs, nfused): programs[i] are the words
+    of step i (None for other steps), nfused[i] the contraction steps it
+    stands for."""
+    reads = defaultdict(int)
+    for kind, a, b, c, *_ in steps:
+        reads[a] += 1
+        if kind == 0 and b != a:
+            reads[b] += 1
+
+    def plain(s):
+        sl = slots[s]
+        return not sl.deps and not sl.carry and not any(sl.slice_stride)
+
+    cand = {}
+    for i, (kind, a, b, c, M, N, K) in enumerate(steps):
+        if kind != 0 or not (dts[a] == dts[b] == dts[c] == _lib.PQ_C64):
+            continue
+        big, op = (a, b) if slots[a].elems >= slots[b].elems else (b, a)
+        if slots[op].elems > op_elems or slots[big].elems < min_elems or slots[c].elems < min_elems:
+            continue
+        if not (plain(a) and plain(b) and plain(c)):
+            continue
+        cand[i] = big
+    # chains: a candidate continues the chain whose last result it reads
+    chains, end_of = [], {}
+    for i in sorted(cand):
+        big = cand[i]
+        if big in end_of and reads[big] == 1:
+            ch = end_of.pop(big)
+        else:
+            ch = []
+            chains.append(ch)
+        ch.append(i)
+        end_of[steps[i][3]] = ch
+    groups = []
+    for ch in chains:
+        i = 0
+        while i < len(ch):
+            j, best = i + 1, None
+            while j < len(ch) and j - i < _lib.PQ_CHAIN_MAX_STAGES:
+                w = chain_program(slots, steps, ch[i:j + 1], tile_bits)
+                if w is None:
+                    break
+                best = (j, w)
+                j += 1
+            if best is None:
+                i += 1
+                continue
+            groups.append((ch[i:best[0] + 1], best[1]))
+            i = best[0] + 1
+    if not groups:
+        return steps, [None] * len(steps), [1 if s[0] == 0 else 0 for s in steps]
+    drop, at = set(), {}
+    for members, words in groups:
+        drop |= set(members[:-1])
+        at[members[-1]] = (members, words)
+    out, progs, nf = [], [], []
+    for i, st in enumerate(steps):
+        if i in drop:
+            continue
+        if i in at:
+            members, words = at[i]
+            first = steps[members[0]]
+            x0 = first[1] if slots[first[1]].elems >= slots[first[2]].elems else first[2]
+            op0 = first[2] if x0 == first[1] else first[1]
+            out.append((2, x0, op0, st[3], [], [], []))
+            progs.append(words)
+            nf.append(len(members))
+        else:
+            out.append(st)
+            progs.append(None)
+            nf.append(1 if st[0] == 0 else 0)
+    return out, progs, nf
+
+
 class Plan:
     """A compiled plan on a context.  execute() enqueues one full run."""
 
@@ -353,7 +564,7 @@ def choose_group_bits(slots, n_sliced, budget_bytes, esize):
 
 def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=None, tiles=None,
                  budget_bytes=64 << 30, precision="uniform", fp64_steps=(), carry_elems=SMALL_ELEMS,
-                 small_elems=MIXED_FP64_ELEMS):
+                 small_elems=MIXED_FP64_ELEMS, fuse=True):
     """Compile a pair plan of a network whose backend is a HIPBackend.
 
     precision: "uniform" (every step in the backend's type), "mixed" (FP64
@@ -362,7 +573,9 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
     to FP64) or
     "mixed16" (as "mixed", with the complex64 GEMMs on FP16 hi/lo matrix
     cores where their shape allows).  kc / tiles, when given, are per contraction step of
-    `plan` (kernel class and GEMM tile, -1 = automatic)."""
+    `plan` (kernel class and GEMM tile, -1 = automatic).  fuse: runs of
+    complex64 steps that apply small operators to one large tensor become
+    fused-chain steps (fuse_chains; a dict passes its limits)."""
     backend = network.backend
     ctx = backend.ctx
     handles, strides = _leaf_layout(backend)
@@ -377,6 +590,10 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
     if group_bits is None:
         esize = 8 if precision in ("mixed", "mixed16") or ctx.code == _lib.PQ_C64 else 16
         group_bits = choose_group_bits(slots, n_sliced, budget_bytes, esize)
+    if fuse:
+        steps, progs, nfused = fuse_chains(slots, steps, dts, **(fuse if isinstance(fuse, dict) else {}))
+    else:
+        progs, nfused = [None] * len(steps), [1 if st[0] == 0 else 0 for st in steps]
     slot_elems = [s.elems for s in slots]
     slot_mask = [0 if s.leaf else sum(1 << j for j in s.deps) for s in slots]
     slot_leaf = [s.leaf for s in slots]
@@ -388,7 +605,8 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
     tls_in = list(tiles) if tiles is not None else None
     kcs, tls = [], []
     ci = 0
-    for kind, sa, sb, sc, M, N, K in steps:
+    chain_off, words = [], []
+    for (kind, sa, sb, sc, M, N, K), prog, nf in zip(steps, progs, nfused):
         step_slots += [sa, sb, sc]
         nmodes += [len(M), len(N), len(K)]
         kinds.append(kind)
@@ -396,22 +614,27 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
         for grp in (M, N, K):
             for m in grp:
                 modes += list(m)
+        chain_off.append(len(words) if prog is not None else -1)
+        if prog is not None:
+            words += prog
         if kind == 0:
             kcs.append(kcs_in[ci] if kcs_in else -1)
             tls.append(tls_in[ci] if tls_in else -1)
-            ci += 1
         else:
             kcs.append(-1)
             tls.append(-1)
+        ci += nf
     keep = dict(
         slot_elems=_arr(ct.c_int64, slot_elems), slot_mask=_arr(ct.c_int64, slot_mask),
         slot_leaf=_arr(ct.c_int64, slot_leaf), lss=_arr(ct.c_int64, lss or [0]),
         step_slots=_arr(ct.c_int32, step_slots), kc=_arr(ct.c_int32, kcs), tile=_arr(ct.c_int32, tls),
         nmodes=_arr(ct.c_int32, nmodes), modes=_arr(ct.c_int64, modes or [0]),
-        dt=_arr(ct.c_int32, dts), kind=_arr(ct.c_int32, kinds), arith=_arr(ct.c_int32, ariths))
+        dt=_arr(ct.c_int32, dts), kind=_arr(ct.c_int32, kinds), arith=_arr(ct.c_int32, ariths),
+        chain_off=_arr(ct.c_int64, chain_off), words=_arr(ct.c_int32, words or [0]))
     d = PlanDesc(len(slots), keep["slot_elems"], keep["slot_mask"], keep["slot_leaf"], n_sliced, keep["lss"],
                  group_bits, len(steps), keep["step_slots"], keep["kc"], keep["tile"], keep["nmodes"],
-                 keep["modes"], out_slot, keep["dt"], keep["kind"], keep["arith"])
+                 keep["modes"], out_slot, keep["dt"], keep["kind"], keep["arith"], keep["chain_off"],
+                 len(words), keep["words"])
     p = ct.c_void_p()
     ctx.check(ctx.lib.pq_plan_create(ctx.ptr, ct.cast(ct.pointer(d), ct.c_void_p), ct.byref(p)))
     return Plan(ctx, p, slots, steps, out_slot, n_sliced, group_bits, dts)

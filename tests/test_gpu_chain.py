@@ -1,0 +1,94 @@
+"""Fused-chain steps (PQ_KC_CHAIN) on the GPU: a plan compiled with and
+without fusion gives the same amplitudes (complex64 tolerance) and the
+oracle's; the chain kernel really ran; bad programs are rejected on the
+host before anything is launched."""
+import numpy as np
+import pytest
+
+from oracle import pq_oracle as O
+import picoquant_hip as P
+from picoquant_hip import _lib, circuits as C, planner as PL
+from picoquant_hip.executor import compile_plan
+
+pytestmark = pytest.mark.gpu
+
+CFG = "0110" + "x" * 4 + "1001" + "01" + "x" * 2
+
+
+def _net():
+    n = 16
+    tn = P.network_from_gates(n, C.grid_circuit_gates(4, 4, 8, 7), backend=P.HIPBackend(np.complex128),
+                              decompose=True)
+    P.add_input(tn, "0" * n)
+    P.add_output(tn, CFG)
+    return tn
+
+
+def _oracle():
+    to = O.network_from_gates(16, O.grid_circuit_gates(4, 4, 8, 7), decompose=True)
+    O.add_input(to, "0" * 16)
+    O.add_output(to, CFG)
+    return to
+
+
+@pytest.mark.parametrize("tile_bits", [6, 10, 12])
+def test_fused_plan_matches_unfused_and_oracle(tile_bits):
+    tn = _net()
+    plan = PL.reconfigure(tn, PL.sweep_plan(tn, PL.grid_site_order(4, 4, "col")), windows=(8,))
+    limits = {"tile_bits": tile_bits, "min_elems": 64, "op_elems": 256}
+    base = compile_plan(tn, plan, precision="mixed", small_elems=16, fuse=False)
+    fused = compile_plan(tn, plan, precision="mixed", small_elems=16, fuse=limits)
+    kinds = [s[0] for s in fused.steps]
+    assert kinds.count(2) >= 1
+    ctx = tn.backend.ctx
+    ctx.profile_enable(True)
+    for use_graph in (False, True, True):
+        fused.execute(use_graph)
+    got = fused.result().reshape(-1, order="F")
+    launches = ctx.profile_read(_lib.KC_CHAIN)["launches"]
+    ctx.profile_enable(False)
+    assert launches >= kinds.count(2)
+    base.execute(True)
+    want = base.result().reshape(-1, order="F")
+    to = _oracle()
+    O.contract_network(to, plan)
+    ref = np.asarray(to.backend.load_tensor_data("result")).reshape(-1, order="F")
+    scale = np.linalg.norm(ref)
+    assert np.linalg.norm(got - want) / scale < 2e-6
+    assert np.linalg.norm(got - ref) / scale < 2e-6
+    for i, k in enumerate(kinds):
+        if k == 2:
+            assert fused.step_info(i)["kc"] == _lib.KC_CHAIN
+    base.close()
+    fused.close()
+
+
+def test_bad_chain_programs_are_rejected():
+    from picoquant_hip import executor as E
+    tn = _net()
+    plan = PL.reconfigure(tn, PL.sweep_plan(tn, PL.grid_site_order(4, 4, "col")), windows=(8,))
+    limits = {"tile_bits": 10, "min_elems": 64, "op_elems": 256}
+    real = E.fuse_chains
+
+    def corrupt(how):
+        def f(slots, steps, dts, **kw):
+            steps, progs, nf = real(slots, steps, dts, **kw)
+            # the last program, so a truncated one runs off the end of the words
+            i = max(i for i, p in enumerate(progs) if p is not None)
+            progs = list(progs)
+            progs[i] = how(list(progs[i]))
+            return steps, progs, nf
+        return f
+
+    cases = [
+        lambda w: [w[0], 14] + w[2:],                       # tile beyond the limit
+        lambda w: w[:5] + [w[5], 1 << 30] + w[7:],          # gathered stride out of range
+        lambda w: w[:-1],                                   # truncated program
+    ]
+    try:
+        for how in cases:
+            E.fuse_chains = corrupt(how)
+            with pytest.raises(_lib.PQError):
+                compile_plan(tn, plan, precision="mixed", small_elems=16, fuse=limits)
+    finally:
+        E.fuse_chains = real
