@@ -35,8 +35,9 @@ __device__ __forceinline__ floatx4 mfma(float a, float b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// Tile <-> tensor copy: element j = tid + 2^lo it; lane tid takes the low
-// tables' entry tid, iteration it the high tables' entry it.
+// Tile <-> tensor copy: element j = tid + 2^lo it (lo <= 9 bits for the
+// CH_THREADS = 512 lanes); lane tid takes the low tables' entry tid,
+// iteration it the high tables' entry it.
 template <bool GATHER>
 __device__ __forceinline__ void move_tile(c64* tile, const c64* X, c64* Y, int32_t base, const int32_t* tab,
                                           int lo, int hi, int lp, int lg, int hp, int hg, int tid) {
@@ -54,7 +55,86 @@ __device__ __forceinline__ void move_tile(c64* tile, const c64* X, c64* Y, int32
   }
 }
 
-__global__ __launch_bounds__(256) void k_chain(const ChainArgs a) {
+// One stage, T[r, m] = sum_k S[m, k] T[r, k], with NKC chunks of 4 K values
+// and NMB blocks of 16 M values (K = 2^nk <= 4 NKC, M = 2^nm <= 16 NMB).
+// Wave w of the CH_THREADS / 64 takes the blocks of 16 r's numbered w,
+// w + CH_THREADS / 64, ...
+template <int NKC, int NMB>
+__device__ __forceinline__ void stage(c64* tile, const ChainStage& g, const int32_t* tab, int wave, int li, int lg) {
+  const int nk = 1 << g.nk, nm = 1 << g.nm;
+  // This lane's operator values (MFMA A operand: row m = 16 mb + li,
+  // column k = 4 kc + lg), zero outside the operator; its K and M tile
+  // offsets (B operand row k = 4 kc + lg; output row m = 16 mb + 4 lg + i);
+  // the tile offset of its column li within a block of 16 r's.
+  float sr[NKC][NMB], si[NKC][NMB], sn[NKC][NMB];
+  int kof[NKC], mof[NMB][4];
+#pragma unroll
+  for (int kc = 0; kc < NKC; ++kc) {
+    const int k = 4 * kc + lg;
+    const bool kv = k < nk;
+    kof[kc] = kv ? swz(tab[g.kp + k]) : 0;
+#pragma unroll
+    for (int mb = 0; mb < NMB; ++mb) {
+      const int m = 16 * mb + li;
+      c64 v;
+      v.x = 0.f;
+      v.y = 0.f;
+      if (kv && m < nm) v = g.S[tab[g.sk + k] + tab[g.sm + m]];
+      sr[kc][mb] = v.x;
+      si[kc][mb] = v.y;
+      sn[kc][mb] = -v.y;
+    }
+  }
+#pragma unroll
+  for (int mb = 0; mb < NMB; ++mb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = 16 * mb + 4 * lg + i;
+      mof[mb][i] = m < nm ? swz(tab[g.mp + m]) : -1;
+    }
+  const bool rv = li < (1 << g.rlo);
+  const int rlo = swz(tab[g.rl + (rv ? li : 0)]);
+  const bool kpart = NKC == 1 && nk < 4;   // fewer K values than lane groups
+  const int nrb = 1 << g.rhi;
+  for (int rb = wave; rb < nrb; rb += CH_THREADS / 64) {
+    const int roff = rlo ^ swz(tab[g.rh + rb]);
+    // every K value of these r's is read before any M value is written
+    float br[NKC], bi[NKC];
+#pragma unroll
+    for (int kc = 0; kc < NKC; ++kc) {
+      c64 v = tile[roff ^ kof[kc]];
+      if (kpart && lg >= nk) {
+        v.x = 0.f;
+        v.y = 0.f;
+      }
+      br[kc] = v.x;
+      bi[kc] = v.y;
+    }
+#pragma unroll
+    for (int mb = 0; mb < NMB; ++mb) {
+      floatx4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kc = 0; kc < NKC; ++kc) {
+        cr = mfma(sr[kc][mb], br[kc], cr);
+        ci = mfma(sr[kc][mb], bi[kc], ci);
+        cr = mfma(sn[kc][mb], bi[kc], cr);
+        ci = mfma(si[kc][mb], br[kc], ci);
+      }
+      // accumulator register i holds row 4 lg + i, column li
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (rv && mof[mb][i] >= 0) {
+          c64 v;
+          v.x = cr[i];
+          v.y = ci[i];
+          tile[roff ^ mof[mb][i]] = v;
+        }
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
   extern __shared__ c64 tile[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -68,76 +148,16 @@ __global__ __launch_bounds__(256) void k_chain(const ChainArgs a) {
 
   for (int s = 0; s < a.nst; ++s) {
     const ChainStage& g = a.st[s];
-    const int nk = 1 << g.nk, nm = 1 << g.nm;
-    const int nkc = (nk + 3) / 4, nmb = (nm + 15) / 16;
-    // This lane's operator values (MFMA A operand: row m = 16 mb + li,
-    // column k = 4 kc + lg), zero outside the operator; its K and M tile
-    // offsets (B operand row k = 4 kc + lg; output row m = 16 mb + 4 lg + i);
-    // the tile offset of its column li within a block of 16 r's.
-    c64 sv[8][2];
-    float svn[8][2];   // -Im of the operator values
-    int kof[8], mof[2][4];
-#pragma unroll
-    for (int kc = 0; kc < 8; ++kc) {
-      const int k = 4 * kc + lg;
-      const bool kv = kc < nkc && k < nk;
-      kof[kc] = kv ? swz(tab[g.kp + k]) : 0;
-#pragma unroll
-      for (int mb = 0; mb < 2; ++mb) {
-        const int m = 16 * mb + li;
-        c64 v;
-        v.x = 0.f;
-        v.y = 0.f;
-        if (kv && mb < nmb && m < nm) v = g.S[tab[g.sk + k] + tab[g.sm + m]];
-        sv[kc][mb] = v;
-        svn[kc][mb] = -v.y;
-      }
-    }
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int m = 16 * mb + 4 * lg + i;
-        mof[mb][i] = (mb < nmb && m < nm) ? swz(tab[g.mp + m]) : 0;
-      }
-    const bool rv = li < (1 << g.rlo);
-    const int rlo = swz(tab[g.rl + (rv ? li : 0)]);
-    const int nrb = 1 << g.rhi;
-    for (int rb = wave; rb < nrb; rb += 4) {
-      const int roff = rlo ^ swz(tab[g.rh + rb]);
-      // every K value of these r's is read before any M value is written
-      c64 bv[8];
-#pragma unroll
-      for (int kc = 0; kc < 8; ++kc) {
-        c64 v;
-        v.x = 0.f;
-        v.y = 0.f;
-        if (kc < nkc && 4 * kc + lg < nk) v = tile[roff ^ kof[kc]];
-        bv[kc] = v;
-      }
-#pragma unroll
-      for (int mb = 0; mb < 2; ++mb) {
-        if (mb >= nmb) break;
-        floatx4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kc = 0; kc < 8; ++kc) {
-          if (kc >= nkc) break;
-          cr = mfma(sv[kc][mb].x, bv[kc].x, cr);
-          ci = mfma(sv[kc][mb].x, bv[kc].y, ci);
-          cr = mfma(svn[kc][mb], bv[kc].y, cr);
-          ci = mfma(sv[kc][mb].y, bv[kc].x, ci);
-        }
-        // accumulator register i holds row 4 lg + i, column li
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          if (rv && 16 * mb + 4 * lg + i < nm) {
-            c64 v;
-            v.x = cr[i];
-            v.y = ci[i];
-            tile[roff ^ mof[mb][i]] = v;
-          }
-        }
-      }
+    const int nkc = ((1 << g.nk) + 3) / 4, nmb = ((1 << g.nm) + 15) / 16;
+    switch (nkc * 4 + nmb) {
+      case 1 * 4 + 1: stage<1, 1>(tile, g, tab, wave, li, lg); break;
+      case 2 * 4 + 1: stage<2, 1>(tile, g, tab, wave, li, lg); break;
+      case 4 * 4 + 1: stage<4, 1>(tile, g, tab, wave, li, lg); break;
+      case 8 * 4 + 1: stage<8, 1>(tile, g, tab, wave, li, lg); break;
+      case 1 * 4 + 2: stage<1, 2>(tile, g, tab, wave, li, lg); break;
+      case 2 * 4 + 2: stage<2, 2>(tile, g, tab, wave, li, lg); break;
+      case 4 * 4 + 2: stage<4, 2>(tile, g, tab, wave, li, lg); break;
+      default: stage<8, 2>(tile, g, tab, wave, li, lg); break;
     }
     __syncthreads();
   }
@@ -150,6 +170,6 @@ __global__ __launch_bounds__(256) void k_chain(const ChainArgs a) {
 hipError_t launch_chain(const ChainArgs& a, hipStream_t s) {
   if (a.tb < 0 || a.tb > CH_TB || a.nst < 1 || a.nst > CH_ST || a.spec.total < 1) return hipErrorInvalidValue;
   const size_t lds = sizeof(c64) << a.tb;
-  hipLaunchKernelGGL(k_chain, dim3(a.spec.total), dim3(256), lds, s, a);
+  hipLaunchKernelGGL(k_chain, dim3(a.spec.total), dim3(CH_THREADS), lds, s, a);
   return hipGetLastError();
 }

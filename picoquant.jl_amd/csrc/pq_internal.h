@@ -83,12 +83,14 @@ int reduce_blocks_for(uint32_t ktotal);
 // of kind 2; the program format is documented with PQ_CHAIN_* in pq_hip.h).
 // The host expands each program into int32 address tables (logical tile
 // addresses and element offsets) in one device buffer; a bit list of n bits
-// is split into a low part (up to 8 bits: gather / scatter, 4 bits: R) taken
+// is split into a low part (up to CH_LO_BITS: gather / scatter, 4 bits: R) taken
 // from the lane and a high part taken from the loop counter, each with its
 // own table, so the kernel only adds table entries.
 #define CH_TB 13   // PQ_CHAIN_MAX_TILE_BITS
 #define CH_OB 5    // PQ_CHAIN_MAX_OP_BITS
 #define CH_ST 8    // PQ_CHAIN_MAX_STAGES
+#define CH_THREADS 512   // 8 waves share one tile (two 64 KiB tiles fill a CU's LDS)
+#define CH_LO_BITS 9     // gather / scatter bits taken from the lane (2^9 = CH_THREADS)
 struct ChainStage {
   const c64* S;
   int32_t nk, nm;          // 2^nk operator columns, 2^nm rows

@@ -267,9 +267,9 @@ const char* parse_chain(pq_ctx* c, pq_plan* p, const pq_plan_desc* d, int i, PSt
   if (!slot_holds(c, X, xspan)) return "gather out of range";
   if (!slot_holds(c, Y, yspan)) return "scatter out of range";
   if (((int64_t)a.spec.total << nout) != Y.elems) return "scatter does not cover the output";
-  a.ilo = std::min(nin, 8);
+  a.ilo = std::min(nin, CH_LO_BITS);
   a.ihi = nin - a.ilo;
-  a.olo = std::min(nout, 8);
+  a.olo = std::min(nout, CH_LO_BITS);
   a.ohi = nout - a.olo;
   a.ilp = bit_table(tab, ipos, 0, a.ilo, true);
   a.ilg = bit_table(tab, istr, 0, a.ilo, false);
