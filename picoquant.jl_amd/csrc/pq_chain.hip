@@ -60,7 +60,8 @@ __device__ __forceinline__ void move_tile(c64* tile, const c64* X, c64* Y, int32
 // Wave w of the CH_THREADS / 64 takes the blocks of 16 r's numbered w,
 // w + CH_THREADS / 64, ...
 template <int NKC, int NMB>
-__device__ __forceinline__ void stage(c64* tile, const ChainStage& g, const int32_t* tab, int wave, int li, int lg) {
+__device__ __forceinline__ void stage(c64* tile, const c64* sd, const int* si_, const ChainStage& g, const int32_t* tab,
+                                      int wave, int li, int lg) {
   const int nk = 1 << g.nk, nm = 1 << g.nm;
   // This lane's operator values (MFMA A operand: row m = 16 mb + li,
   // column k = 4 kc + lg), zero outside the operator; its K and M tile
@@ -68,18 +69,19 @@ __device__ __forceinline__ void stage(c64* tile, const ChainStage& g, const int3
   // the tile offset of its column li within a block of 16 r's.
   float sr[NKC][NMB], si[NKC][NMB], sn[NKC][NMB];
   int kof[NKC], mof[NMB][4];
+  // operator values and offsets come from the LDS staging area
 #pragma unroll
   for (int kc = 0; kc < NKC; ++kc) {
     const int k = 4 * kc + lg;
     const bool kv = k < nk;
-    kof[kc] = kv ? swz(tab[g.kp + k]) : 0;
+    kof[kc] = kv ? si_[g.ko + k] : 0;
 #pragma unroll
     for (int mb = 0; mb < NMB; ++mb) {
       const int m = 16 * mb + li;
       c64 v;
       v.x = 0.f;
       v.y = 0.f;
-      if (kv && m < nm) v = g.S[tab[g.sk + k] + tab[g.sm + m]];
+      if (kv && m < nm) v = sd[g.sd + m * nk + k];
       sr[kc][mb] = v.x;
       si[kc][mb] = v.y;
       sn[kc][mb] = -v.y;
@@ -90,14 +92,20 @@ __device__ __forceinline__ void stage(c64* tile, const ChainStage& g, const int3
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int m = 16 * mb + 4 * lg + i;
-      mof[mb][i] = m < nm ? swz(tab[g.mp + m]) : -1;
+      mof[mb][i] = m < nm ? si_[g.mo + m] : -1;
     }
   const bool rv = li < (1 << g.rlo);
-  const int rlo = swz(tab[g.rl + (rv ? li : 0)]);
+  const int rlo = si_[g.ro + (rv ? li : 0)];
   const bool kpart = NKC == 1 && nk < 4;   // fewer K values than lane groups
+  // every lane stores all its outputs (no partial R or M block)
+  const bool full = (1 << g.rlo) == 16 && nm >= 16 * NMB;
   const int nrb = 1 << g.rhi;
-  for (int rb = wave; rb < nrb; rb += CH_THREADS / 64) {
-    const int roff = rlo ^ swz(tab[g.rh + rb]);
+  constexpr int NW = CH_THREADS / 64;
+  // the next block's high R offset is fetched one iteration ahead
+  int rh = wave < nrb ? tab[g.rh + wave] : 0;
+  for (int rb = wave; rb < nrb; rb += NW) {
+    const int roff = rlo ^ swz(rh);
+    if (rb + NW < nrb) rh = tab[g.rh + rb + NW];
     // every K value of these r's is read before any M value is written
     float br[NKC], bi[NKC];
 #pragma unroll
@@ -123,7 +131,7 @@ __device__ __forceinline__ void stage(c64* tile, const ChainStage& g, const int3
       // accumulator register i holds row 4 lg + i, column li
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        if (rv && mof[mb][i] >= 0) {
+        if (full || (rv && mof[mb][i] >= 0)) {
           c64 v;
           v.x = cr[i];
           v.y = ci[i];
@@ -143,6 +151,21 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
   int32_t ox, oy;
   group_offsets(a.spec, blockIdx.x, ox, oy);
 
+  // stage every operator (dense [m][k]) and the swizzled K / M / low-R
+  // offsets in LDS after the tile; the loads overlap the gather's
+  c64* sd = tile + (1 << a.tb);
+  int* sidx = reinterpret_cast<int*>(sd + a.sdn);
+  for (int s = 0; s < a.nst; ++s) {
+    const ChainStage& g = a.st[s];
+    const int nk = 1 << g.nk, nkm = 1 << (g.nk + g.nm);
+    for (int e = tid; e < nkm; e += CH_THREADS) {
+      const int m = e >> g.nk, k = e & (nk - 1);
+      sd[g.sd + e] = g.S[tab[g.sk + k] + tab[g.sm + m]];
+    }
+    for (int e = tid; e < nk; e += CH_THREADS) sidx[g.ko + e] = swz(tab[g.kp + e]);
+    for (int e = tid; e < (1 << g.nm); e += CH_THREADS) sidx[g.mo + e] = swz(tab[g.mp + e]);
+    for (int e = tid; e < (1 << g.rlo); e += CH_THREADS) sidx[g.ro + e] = swz(tab[g.rl + e]);
+  }
   move_tile<true>(tile, a.X, nullptr, ox, tab, a.ilo, a.ihi, a.ilp, a.ilg, a.ihp, a.ihg, tid);
   __syncthreads();
 
@@ -150,14 +173,14 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
     const ChainStage& g = a.st[s];
     const int nkc = ((1 << g.nk) + 3) / 4, nmb = ((1 << g.nm) + 15) / 16;
     switch (nkc * 4 + nmb) {
-      case 1 * 4 + 1: stage<1, 1>(tile, g, tab, wave, li, lg); break;
-      case 2 * 4 + 1: stage<2, 1>(tile, g, tab, wave, li, lg); break;
-      case 4 * 4 + 1: stage<4, 1>(tile, g, tab, wave, li, lg); break;
-      case 8 * 4 + 1: stage<8, 1>(tile, g, tab, wave, li, lg); break;
-      case 1 * 4 + 2: stage<1, 2>(tile, g, tab, wave, li, lg); break;
-      case 2 * 4 + 2: stage<2, 2>(tile, g, tab, wave, li, lg); break;
-      case 4 * 4 + 2: stage<4, 2>(tile, g, tab, wave, li, lg); break;
-      default: stage<8, 2>(tile, g, tab, wave, li, lg); break;
+      case 1 * 4 + 1: stage<1, 1>(tile, sd, sidx, g, tab, wave, li, lg); break;
+      case 2 * 4 + 1: stage<2, 1>(tile, sd, sidx, g, tab, wave, li, lg); break;
+      case 4 * 4 + 1: stage<4, 1>(tile, sd, sidx, g, tab, wave, li, lg); break;
+      case 8 * 4 + 1: stage<8, 1>(tile, sd, sidx, g, tab, wave, li, lg); break;
+      case 1 * 4 + 2: stage<1, 2>(tile, sd, sidx, g, tab, wave, li, lg); break;
+      case 2 * 4 + 2: stage<2, 2>(tile, sd, sidx, g, tab, wave, li, lg); break;
+      case 4 * 4 + 2: stage<4, 2>(tile, sd, sidx, g, tab, wave, li, lg); break;
+      default: stage<8, 2>(tile, sd, sidx, g, tab, wave, li, lg); break;
     }
     __syncthreads();
   }
@@ -169,7 +192,8 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
 
 hipError_t launch_chain(const ChainArgs& a, hipStream_t s) {
   if (a.tb < 0 || a.tb > CH_TB || a.nst < 1 || a.nst > CH_ST || a.spec.total < 1) return hipErrorInvalidValue;
-  const size_t lds = sizeof(c64) << a.tb;
+  if ((int64_t)a.sdn * 8 + (int64_t)a.idn * 4 > CH_STAGE_BYTES) return hipErrorInvalidValue;
+  const size_t lds = (sizeof(c64) << a.tb) + sizeof(c64) * a.sdn + sizeof(int) * a.idn;
   hipLaunchKernelGGL(k_chain, dim3(a.spec.total), dim3(CH_THREADS), lds, s, a);
   return hipGetLastError();
 }

@@ -2,6 +2,7 @@
 // the kernel launchers.
 #pragma once
 #include "pq_common.h"
+#include "../../include/pq_hip.h"
 #include <string>
 
 // Big x small pairwise contraction: X is the large operand, Y the small one
@@ -97,6 +98,8 @@ struct ChainStage {
   int32_t rlo, rhi;        // R bits from the lane (<= 4) and from the block counter
   int32_t kp, mp, sk, sm;  // table offsets: K / M tile addresses, K / M operator offsets
   int32_t rl, rh;          // table offsets: R tile addresses (low, high part)
+  int32_t sd, ko, mo, ro;  // LDS staging: dense operator [m][k] (c64 units), K / M / low-R
+                           // swizzled tile offsets (int units)
 };
 struct ChainArgs {
   const c64* X;
@@ -106,7 +109,9 @@ struct ChainArgs {
   int32_t ilo, ihi, olo, ohi;              // gather / scatter bits from the lane, from the counter
   int32_t ilp, ilg, ihp, ihg;              // gather tables: low / high tile address, element offset
   int32_t olp, olg, ohp, ohg;              // scatter tables
+  int32_t sdn, idn;                        // staged c64 operator elements, staged ints
   Group spec;   // spectator modes: s0 = stride in X, s1 = stride in Y
   ChainStage st[CH_ST];
 };
+#define CH_STAGE_BYTES PQ_CHAIN_STAGE_BYTES   // LDS after the tile for the staged operators and offsets
 hipError_t launch_chain(const ChainArgs& a, hipStream_t s);

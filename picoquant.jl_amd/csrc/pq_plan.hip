@@ -312,6 +312,13 @@ const char* parse_chain(pq_ctx* c, pq_plan* p, const pq_plan_desc* d, int i, PSt
     g.sm = bit_table(tab, mstr, 0, nm, false);
     g.rl = bit_table(tab, rpos, 0, g.rlo, true);
     g.rh = bit_table(tab, rpos, g.rlo, nr, true);
+    g.sd = a.sdn;
+    a.sdn += 1 << (nk + nm);
+    g.ko = a.idn;
+    g.mo = a.idn + (1 << nk);
+    g.ro = a.idn + (1 << nk) + (1 << nm);
+    a.idn += (1 << nk) + (1 << nm) + (1 << g.rlo);
+    if ((int64_t)a.sdn * 8 + (int64_t)a.idn * 4 > CH_STAGE_BYTES) return "fused-chain operators exceed the LDS staging area";
     st.ops.push_back(os);
     st.level = std::max(st.level, 1 + (S.leaf ? 0 : p->steps[S.producer].level));
     p->slots[os].last_use = i;

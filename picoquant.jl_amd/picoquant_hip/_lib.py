@@ -33,6 +33,7 @@ KC_NAMES = {KC_STREAM: "stream", KC_GEMM: "gemm", KC_GENERIC: "generic", KC_REDU
 PQ_CHAIN_MAX_STAGES = 8
 PQ_CHAIN_MAX_TILE_BITS = 13
 PQ_CHAIN_MAX_OP_BITS = 5
+PQ_CHAIN_STAGE_BYTES = 16384
 
 # Every exported symbol, with (restype, argtypes).  tests/test_boundary.py
 # checks this list against include/pq_hip.h.

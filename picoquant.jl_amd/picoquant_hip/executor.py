@@ -378,6 +378,11 @@ def chain_program(slots, steps, group, tile_bits=CHAIN_TILE_BITS):
         words_st.append((op, kpos, mpos, rbits))
     if top > tile_bits:
         return None
+    # operators and per-stage offsets staged in LDS beside the tile
+    staged = sum(8 * (1 << (len(kp) + len(mp))) + 4 * ((1 << len(kp)) + (1 << len(mp)) + (1 << min(len(rb), 4)))
+                 for _, kp, mp, rb in words_st)
+    if staged > _lib.PQ_CHAIN_STAGE_BYTES:
+        return None
     final = {bit[0] for bit in pos}
     scatter = sorted(((pos[bit], s) for bit, s in _bits(Y, [l for l in Y.idx if l in final])), key=lambda t: t[1])
     if len(scatter) != len(pos):
