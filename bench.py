@@ -214,28 +214,30 @@ def cpu_baseline(plan, sliced, cap_s=20.0):
             "elapsed_s": el}
 
 
-TRAFFIC_FILE = ROOT / "profiles" / "gemm_grid6x6d14_traffic.json"
+PROFILES = ROOT / "profiles"
 
 
 def measured_traffic(kernel_class, d):
     """(HBM bytes per launch, detail) of the dominant kernel class from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by the
-    copy-kernel calibration), or (None, None) when none is committed."""
-    if not TRAFFIC_FILE.exists():
-        return None, None
-    t = json.loads(TRAFFIC_FILE.read_text())
+    copy-kernel calibration); None with the reason in the detail when no
+    summary for this class and plan is committed."""
+    traffic_file = PROFILES / f"{kernel_class}_grid6x6d14_traffic.json"
+    if not traffic_file.exists():
+        return None, {"missing": "no profiles/" + traffic_file.name, "alg_bytes_per_run": d["bytes"]}
+    t = json.loads(traffic_file.read_text())
     if t.get("kernel_class") != kernel_class:
         return None, None
     # the summary holds for the plan it was measured on: same class bytes
     prof_alg = t.get("class_alg_bytes_per_run")
     if not prof_alg or abs(prof_alg - d["bytes"]) > 0.01 * d["bytes"]:
-        return None, {"stale": "profiles/" + TRAFFIC_FILE.name + " was measured on another plan",
+        return None, {"stale": "profiles/" + traffic_file.name + " was measured on another plan",
                       "alg_bytes_per_run": d["bytes"], "profiled_alg_bytes_per_run": prof_alg}
-    per_launch = t["hbm_bytes_per_run"] / t["gemm_launches_per_run"]
+    per_launch = t["hbm_bytes_per_run"] / t["launches_per_run"]
     alg_per_launch = d["bytes"] / d["steps"]
     return per_launch, {"hbm_bytes_per_run": t["hbm_bytes_per_run"], "alg_bytes_per_run": d["bytes"],
                         "hbm_over_alg": t["hbm_bytes_per_run"] / d["bytes"],
-                        "alg_bytes_per_launch": alg_per_launch, "source": "profiles/" + TRAFFIC_FILE.name}
+                        "alg_bytes_per_launch": alg_per_launch, "source": "profiles/" + traffic_file.name}
 
 
 def calibrate_copy(mib):
