@@ -204,9 +204,10 @@ typedef struct {
 #define PQ_CHAIN_MAX_STAGES 8
 #define PQ_CHAIN_MAX_TILE_BITS 13
 #define PQ_CHAIN_MAX_OP_BITS 5
-/* A chain's operators (8 bytes per element) and its per-stage K, M and low-R
- * offsets (4 bytes each; 2^NK + 2^NM + 2^min(NR, 4) per stage) are staged in
- * LDS beside the tile; together they may take this many bytes. */
+/* A chain's operators (8 bytes per element) and its per-stage K, M and R
+ * offsets (4 bytes each; 2^NK + 2^NM + 2^min(NR, 4) + 2^(NR - min(NR, 4))
+ * per stage) are staged in LDS beside the tiles; together they may take this
+ * many bytes. */
 #define PQ_CHAIN_STAGE_BYTES 16384
 
 int pq_plan_create(pq_ctx* ctx, const pq_plan_desc* desc, pq_plan** out);

@@ -5,15 +5,21 @@
 // elements) with the previous step's result rewrites a few bits of one big
 // tensor per step and is HBM-bound: every step reads and writes the whole
 // tensor.  Here the tensor's modes are split into bits; the bits any
-// operator touches form a tile of at most 2^13 elements.  One workgroup per
-// configuration of the other ("spectator") modes:
-//   1. gathers its tile from X into LDS (bits ordered by X stride, so
-//      adjacent lanes read adjacent addresses),
-//   2. applies the operators in order, in place:
+// operator touches form a tile of at most 2^13 elements, one tile per
+// configuration of the other ("spectator") modes.  For each tile:
+//   1. gather it from X into LDS (bits ordered by X stride, so adjacent
+//      lanes read adjacent addresses),
+//   2. apply the operators in order, in place:
 //        T[r, m] = sum_k S_t[m, k] T[r, k]
 //      as 16 x 16 x 4 complex blocks on v_mfma_f32_16x16x4_f32 (4 real
 //      MFMAs per block: Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br),
-//   3. scatters the tile to Y (bits ordered by Y stride).
+//   3. scatter it to Y (bits ordered by Y stride).
+// Workgroups are persistent (two per CU, 8 waves each, one tile buffer
+// each): while the stages run on one tile, the next tile's loads are in
+// flight into registers, and the previous tile's stores drain, so HBM
+// traffic overlaps the MFMA work instead of every workgroup on the chip
+// gathering, computing and scattering in lockstep; a workgroup waiting at
+// a stage barrier leaves the CU's matrix cores to the other.
 // Tile positions of every bit, per stage, come from the host program
 // (pq_hip.h, PQ_CHAIN_*); in place is safe because a wave reads all K
 // values of its 16 r's before it writes their M values, and different
@@ -24,69 +30,52 @@ namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-// LDS bank spreading: the tile is stored at swz(e), a bijection that is
-// linear over GF(2) (the low 5 bits are XORed with bits 5-9 and 10-14), so
-// swz(a | b) = swz(a) ^ swz(b) for addresses built from disjoint bit sets.
-// 16 lanes whose addresses differ by multiples of any power of two up to
-// 2^11 then fall on 16 distinct 8-byte bank slots.
-__device__ __forceinline__ int swz(int e) { return e ^ ((e >> 5) & 31) ^ ((e >> 10) & 31); }
+// LDS addresses of tile elements come from the host's tables (pq_plan.hip,
+// chain_layout): a linear map over GF(2) chosen per chain so that the lanes
+// of every access below fall on distinct banks; addresses of disjoint bit
+// sets combine by XOR.
 
 __device__ __forceinline__ floatx4 mfma(float a, float b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// Tile <-> tensor copy: element j = tid + 2^lo it (lo <= 9 bits for the
-// CH_THREADS = 512 lanes); lane tid takes the low tables' entry tid,
-// iteration it the high tables' entry it.
-template <bool GATHER>
-__device__ __forceinline__ void move_tile(c64* tile, const c64* X, c64* Y, int32_t base, const int32_t* tab,
-                                          int lo, int hi, int lp, int lg, int hp, int hg, int tid) {
-  if (tid >= (1 << lo)) return;
-  const int p0 = swz(tab[lp + tid]);
-  const int32_t g0 = base + tab[lg + tid];
-  const int nit = 1 << hi;
-  for (int it = 0; it < nit; ++it) {
-    const int p = p0 ^ swz(tab[hp + it]);
-    const int32_t g = g0 + tab[hg + it];
-    if (GATHER)
-      tile[p] = X[g];
-    else
-      Y[g] = tile[p];
-  }
-}
-
 // One stage, T[r, m] = sum_k S[m, k] T[r, k], with NKC chunks of 4 K values
 // and NMB blocks of 16 M values (K = 2^nk <= 4 NKC, M = 2^nm <= 16 NMB).
 // Wave w of the CH_THREADS / 64 takes the blocks of 16 r's numbered w,
-// w + CH_THREADS / 64, ...
+// w + CH_THREADS / 64, ...  The lane's operator values (MFMA A operand:
+// row m = 16 mb + li, column k = 4 kc + lg) stay in registers when there
+// are at most 8 chunk-blocks of them; the 32 x 32 case rereads them from
+// LDS per M block (register room for the next tile's prefetch).
 template <int NKC, int NMB>
-__device__ __forceinline__ void stage(c64* tile, const c64* sd, const int* si_, const ChainStage& g, const int32_t* tab,
-                                      int wave, int li, int lg) {
+__device__ __forceinline__ void stage(c64* tile, const c64* sd, const int* si_, const ChainStage& g, int wave, int li,
+                                      int lg) {
+  constexpr bool HOLD = NKC * NMB <= 8;
+  constexpr int HMB = HOLD ? NMB : 1;
   const int nk = 1 << g.nk, nm = 1 << g.nm;
-  // This lane's operator values (MFMA A operand: row m = 16 mb + li,
-  // column k = 4 kc + lg), zero outside the operator; its K and M tile
-  // offsets (B operand row k = 4 kc + lg; output row m = 16 mb + 4 lg + i);
-  // the tile offset of its column li within a block of 16 r's.
-  float sr[NKC][NMB], si[NKC][NMB], sn[NKC][NMB];
+  auto opval = [&](int kc, int mb) -> c64 {
+    const int k = 4 * kc + lg, m = 16 * mb + li;
+    c64 v;
+    v.x = 0.f;
+    v.y = 0.f;
+    if (k < nk && m < nm) v = sd[g.sd + k * nm + m];
+    return v;
+  };
+  float sr[NKC][HMB], si[NKC][HMB];
   int kof[NKC], mof[NMB][4];
-  // operator values and offsets come from the LDS staging area
 #pragma unroll
   for (int kc = 0; kc < NKC; ++kc) {
     const int k = 4 * kc + lg;
-    const bool kv = k < nk;
-    kof[kc] = kv ? si_[g.ko + k] : 0;
+    kof[kc] = k < nk ? si_[g.ko + k] : 0;
+    if constexpr (HOLD) {
 #pragma unroll
-    for (int mb = 0; mb < NMB; ++mb) {
-      const int m = 16 * mb + li;
-      c64 v;
-      v.x = 0.f;
-      v.y = 0.f;
-      if (kv && m < nm) v = sd[g.sd + m * nk + k];
-      sr[kc][mb] = v.x;
-      si[kc][mb] = v.y;
-      sn[kc][mb] = -v.y;
+      for (int mb = 0; mb < NMB; ++mb) {
+        const c64 v = opval(kc, mb);
+        sr[kc][mb] = v.x;
+        si[kc][mb] = v.y;
+      }
     }
   }
+  // output row m = 16 mb + 4 lg + i of the lane's accumulator register i
 #pragma unroll
   for (int mb = 0; mb < NMB; ++mb)
 #pragma unroll
@@ -101,11 +90,8 @@ __device__ __forceinline__ void stage(c64* tile, const c64* sd, const int* si_, 
   const bool full = (1 << g.rlo) == 16 && nm >= 16 * NMB;
   const int nrb = 1 << g.rhi;
   constexpr int NW = CH_THREADS / 64;
-  // the next block's high R offset is fetched one iteration ahead
-  int rh = wave < nrb ? tab[g.rh + wave] : 0;
   for (int rb = wave; rb < nrb; rb += NW) {
-    const int roff = rlo ^ swz(rh);
-    if (rb + NW < nrb) rh = tab[g.rh + rb + NW];
+    const int roff = rlo ^ si_[g.rho + rb];
     // every K value of these r's is read before any M value is written
     float br[NKC], bi[NKC];
 #pragma unroll
@@ -120,15 +106,24 @@ __device__ __forceinline__ void stage(c64* tile, const c64* sd, const int* si_, 
     }
 #pragma unroll
     for (int mb = 0; mb < NMB; ++mb) {
+      if constexpr (!HOLD) {
+#pragma unroll
+        for (int kc = 0; kc < NKC; ++kc) {
+          const c64 v = opval(kc, mb);
+          sr[kc][0] = v.x;
+          si[kc][0] = v.y;
+        }
+      }
+      constexpr int hm = 0;
       floatx4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int kc = 0; kc < NKC; ++kc) {
-        cr = mfma(sr[kc][mb], br[kc], cr);
-        ci = mfma(sr[kc][mb], bi[kc], ci);
-        cr = mfma(sn[kc][mb], bi[kc], cr);
-        ci = mfma(si[kc][mb], br[kc], ci);
+        const float ar = sr[kc][HOLD ? mb : hm], ai = si[kc][HOLD ? mb : hm];
+        cr = mfma(ar, br[kc], cr);
+        ci = mfma(ar, bi[kc], ci);
+        cr = mfma(-ai, bi[kc], cr);
+        ci = mfma(ai, br[kc], ci);
       }
-      // accumulator register i holds row 4 lg + i, column li
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         if (full || (rv && mof[mb][i] >= 0)) {
@@ -143,57 +138,137 @@ __device__ __forceinline__ void stage(c64* tile, const c64* sd, const int* si_, 
 }
 
 __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
-  extern __shared__ c64 tile[];
+  extern __shared__ c64 smem[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int li = lane & 15, lg = lane >> 4;
   const int32_t* tab = a.tab;
-  int32_t ox, oy;
-  group_offsets(a.spec, blockIdx.x, ox, oy);
+  const uint32_t total = a.spec.total, G = gridDim.x;
+  c64* tile = smem;
 
-  // stage every operator (dense [m][k]) and the swizzled K / M / low-R
-  // offsets in LDS after the tile; the loads overlap the gather's
-  c64* sd = tile + (1 << a.tb);
+  // stage every operator (dense [k][m]: the 16 lanes of a lane group read
+  // 16 consecutive m) and the K / M / R tile addresses in LDS after the
+  // tile, once per workgroup
+  c64* sd = smem + (1 << a.tb);
   int* sidx = reinterpret_cast<int*>(sd + a.sdn);
   for (int s = 0; s < a.nst; ++s) {
     const ChainStage& g = a.st[s];
     const int nk = 1 << g.nk, nkm = 1 << (g.nk + g.nm);
     for (int e = tid; e < nkm; e += CH_THREADS) {
-      const int m = e >> g.nk, k = e & (nk - 1);
+      const int k = e >> g.nm, m = e & ((1 << g.nm) - 1);
       sd[g.sd + e] = g.S[tab[g.sk + k] + tab[g.sm + m]];
     }
-    for (int e = tid; e < nk; e += CH_THREADS) sidx[g.ko + e] = swz(tab[g.kp + e]);
-    for (int e = tid; e < (1 << g.nm); e += CH_THREADS) sidx[g.mo + e] = swz(tab[g.mp + e]);
-    for (int e = tid; e < (1 << g.rlo); e += CH_THREADS) sidx[g.ro + e] = swz(tab[g.rl + e]);
+    for (int e = tid; e < nk; e += CH_THREADS) sidx[g.ko + e] = tab[g.kp + e];
+    for (int e = tid; e < (1 << g.nm); e += CH_THREADS) sidx[g.mo + e] = tab[g.mp + e];
+    for (int e = tid; e < (1 << g.rlo); e += CH_THREADS) sidx[g.ro + e] = tab[g.rl + e];
+    for (int e = tid; e < (1 << g.rhi); e += CH_THREADS) sidx[g.rho + e] = tab[g.rh + e];
   }
-  move_tile<true>(tile, a.X, nullptr, ox, tab, a.ilo, a.ihi, a.ilp, a.ilg, a.ihp, a.ihg, tid);
+  // Gather / scatter without branches: element j = lane + 2^lo it of the
+  // tile; each lane does CH_NIT iterations, and the staged iteration
+  // tables wrap around (entry it holds iteration it mod 2^hi), as do the
+  // lanes beyond 2^lo.  A repeated iteration rereads the same element into
+  // the same tile position, or stores the same value to the same address.
+  int* hid = sidx + a.idn;   // tile address / tensor offset per iteration
+  if (tid < CH_NIT) {
+    hid[tid] = tab[a.ihp + (tid & ((1 << a.ihi) - 1))];
+    hid[CH_NIT + tid] = tab[a.ihg + (tid & ((1 << a.ihi) - 1))];
+    hid[2 * CH_NIT + tid] = tab[a.ohp + (tid & ((1 << a.ohi) - 1))];
+    hid[3 * CH_NIT + tid] = tab[a.ohg + (tid & ((1 << a.ohi) - 1))];
+  }
+  __syncthreads();
+  const int gt = tid & ((1 << a.ilo) - 1), st = tid & ((1 << a.olo) - 1);
+  const int gp0 = tab[a.ilp + gt], sp0 = tab[a.olp + st];
+  const int32_t gg0 = tab[a.ilg + gt], sg0 = tab[a.olg + st];
+  c64 P[CH_NIT];   // the next tile, in flight
+  auto load = [&](uint32_t t) {
+    int32_t ox, oy;
+    group_offsets(a.spec, t, ox, oy);
+    const c64* src = a.X + ox + gg0;
+    int o[CH_NIT];
+#pragma unroll
+    for (int it = 0; it < CH_NIT; ++it) o[it] = hid[CH_NIT + it];
+#pragma unroll
+    for (int it = 0; it < CH_NIT; ++it) P[it] = src[o[it]];
+  };
+  auto put = [&](c64* buf) {
+#pragma unroll
+    for (int it = 0; it < CH_NIT; ++it) buf[gp0 ^ hid[it]] = P[it];
+  };
+  auto scatter = [&](const c64* buf, uint32_t t) {
+    int32_t ox, oy;
+    group_offsets(a.spec, t, ox, oy);
+    c64* dst = a.Y + oy + sg0;
+    c64 v[CH_NIT];
+    int o[CH_NIT];
+#pragma unroll
+    for (int it = 0; it < CH_NIT; ++it) {
+      v[it] = buf[sp0 ^ hid[2 * CH_NIT + it]];
+      o[it] = hid[3 * CH_NIT + it];
+    }
+#pragma unroll
+    for (int it = 0; it < CH_NIT; ++it) dst[o[it]] = v[it];
+  };
+
+  uint32_t t = blockIdx.x;
+  if (t < total) {
+    load(t);
+    put(tile);
+  }
+  if (t + G < total) load(t + G);
   __syncthreads();
 
-  for (int s = 0; s < a.nst; ++s) {
-    const ChainStage& g = a.st[s];
-    const int nkc = ((1 << g.nk) + 3) / 4, nmb = ((1 << g.nm) + 15) / 16;
-    switch (nkc * 4 + nmb) {
-      case 1 * 4 + 1: stage<1, 1>(tile, sd, sidx, g, tab, wave, li, lg); break;
-      case 2 * 4 + 1: stage<2, 1>(tile, sd, sidx, g, tab, wave, li, lg); break;
-      case 4 * 4 + 1: stage<4, 1>(tile, sd, sidx, g, tab, wave, li, lg); break;
-      case 8 * 4 + 1: stage<8, 1>(tile, sd, sidx, g, tab, wave, li, lg); break;
-      case 1 * 4 + 2: stage<1, 2>(tile, sd, sidx, g, tab, wave, li, lg); break;
-      case 2 * 4 + 2: stage<2, 2>(tile, sd, sidx, g, tab, wave, li, lg); break;
-      case 4 * 4 + 2: stage<4, 2>(tile, sd, sidx, g, tab, wave, li, lg); break;
-      default: stage<8, 2>(tile, sd, sidx, g, tab, wave, li, lg); break;
+  // every wave of a workgroup runs the same iterations, and the loop ends
+  // when the workgroup's tiles run out
+  for (; t < total; t += G) {
+    for (int s = 0; s < a.nst; ++s) {
+      const ChainStage& g = a.st[s];
+      const int nkc = ((1 << g.nk) + 3) / 4, nmb = ((1 << g.nm) + 15) / 16;
+      switch (nkc * 4 + nmb) {
+        case 1 * 4 + 1: stage<1, 1>(tile, sd, sidx, g, wave, li, lg); break;
+        case 2 * 4 + 1: stage<2, 1>(tile, sd, sidx, g, wave, li, lg); break;
+        case 4 * 4 + 1: stage<4, 1>(tile, sd, sidx, g, wave, li, lg); break;
+        case 8 * 4 + 1: stage<8, 1>(tile, sd, sidx, g, wave, li, lg); break;
+        case 1 * 4 + 2: stage<1, 2>(tile, sd, sidx, g, wave, li, lg); break;
+        case 2 * 4 + 2: stage<2, 2>(tile, sd, sidx, g, wave, li, lg); break;
+        case 4 * 4 + 2: stage<4, 2>(tile, sd, sidx, g, wave, li, lg); break;
+        default: stage<8, 2>(tile, sd, sidx, g, wave, li, lg); break;
+      }
+      __syncthreads();
     }
+    scatter(tile, t);
+    __syncthreads();
+    // the next tile's loads were issued before this tile's stages
+    if (t + G < total) put(tile);
+    if (t + 2 * G < total) load(t + 2 * G);
     __syncthreads();
   }
-
-  move_tile<false>(tile, nullptr, a.Y, oy, tab, a.olo, a.ohi, a.olp, a.olg, a.ohp, a.ohg, tid);
 }
 
 }  // namespace
 
 hipError_t launch_chain(const ChainArgs& a, hipStream_t s) {
   if (a.tb < 0 || a.tb > CH_TB || a.nst < 1 || a.nst > CH_ST || a.spec.total < 1) return hipErrorInvalidValue;
+  if (a.ilo > CH_LO_BITS || a.olo > CH_LO_BITS || a.ihi > CH_TB - CH_LO_BITS || a.ohi > CH_TB - CH_LO_BITS)
+    return hipErrorInvalidValue;
   if ((int64_t)a.sdn * 8 + (int64_t)a.idn * 4 > CH_STAGE_BYTES) return hipErrorInvalidValue;
-  const size_t lds = (sizeof(c64) << a.tb) + sizeof(c64) * a.sdn + sizeof(int) * a.idn;
-  hipLaunchKernelGGL(k_chain, dim3(a.spec.total), dim3(CH_THREADS), lds, s, a);
+  const size_t lds = (sizeof(c64) << a.tb) + sizeof(c64) * a.sdn + sizeof(int) * (a.idn + 4 * CH_NIT);
+  // persistent workgroups: as many as fit on the device at once with this
+  // LDS size (two per CU for a 2^13-element tile)
+  static int ncu[64] = {0};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  if (ncu[dev] == 0) {
+    e = hipDeviceGetAttribute(&ncu[dev], hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    if (ncu[dev] < 1) ncu[dev] = 1;
+  }
+  int per_cu = 0;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_chain, CH_THREADS, lds);
+  if (e != hipSuccess) return e;
+  per_cu = std::max(per_cu, 1);
+  const uint32_t grid = std::min<uint32_t>(a.spec.total, (uint32_t)(per_cu * ncu[dev]));
+  hipLaunchKernelGGL(k_chain, dim3(grid), dim3(CH_THREADS), lds, s, a);
   return hipGetLastError();
 }

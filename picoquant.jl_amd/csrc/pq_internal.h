@@ -90,16 +90,17 @@ int reduce_blocks_for(uint32_t ktotal);
 #define CH_TB 13   // PQ_CHAIN_MAX_TILE_BITS
 #define CH_OB 5    // PQ_CHAIN_MAX_OP_BITS
 #define CH_ST 8    // PQ_CHAIN_MAX_STAGES
-#define CH_THREADS 512   // 8 waves share one tile (two 64 KiB tiles fill a CU's LDS)
+#define CH_THREADS 512   // 8 waves per workgroup, two persistent workgroups per CU
 #define CH_LO_BITS 9     // gather / scatter bits taken from the lane (2^9 = CH_THREADS)
+#define CH_NIT (1 << (CH_TB - CH_LO_BITS))   // gather / scatter iterations per lane and tile
 struct ChainStage {
   const c64* S;
   int32_t nk, nm;          // 2^nk operator columns, 2^nm rows
   int32_t rlo, rhi;        // R bits from the lane (<= 4) and from the block counter
   int32_t kp, mp, sk, sm;  // table offsets: K / M tile addresses, K / M operator offsets
   int32_t rl, rh;          // table offsets: R tile addresses (low, high part)
-  int32_t sd, ko, mo, ro;  // LDS staging: dense operator [m][k] (c64 units), K / M / low-R
-                           // swizzled tile offsets (int units)
+  int32_t sd, ko, mo, ro, rho;  // LDS staging: dense operator [m][k] (c64 units), K / M /
+                                // low-R / high-R tile addresses (int units)
 };
 struct ChainArgs {
   const c64* X;

@@ -187,18 +187,89 @@ bool slot_holds(pq_ctx* c, const Slot& sl, int64_t span) {
   return span <= full;
 }
 
-// Appends the address table of a bit list (bits [b0, b1), positions and
-// strides) to tab: entry t is the sum over the set bits of t of
-// 2^position (pos = true) or of the stride.  Returns the table's offset.
-int32_t bit_table(std::vector<int32_t>& tab, const std::vector<int32_t>& vals, int b0, int b1, bool pos) {
+// Appends the address table of a bit list (bits [b0, b1)) to tab: entry t
+// is, over the set bits b of t, the XOR of lay[position of b] (pos = true:
+// an LDS tile address) or the sum of the strides (pos = false: a tensor
+// offset).  Returns the table's offset.
+int32_t bit_table(std::vector<int32_t>& tab, const std::vector<int32_t>& vals, int b0, int b1, bool pos,
+                  const std::vector<int32_t>& lay) {
   const int32_t off = (int32_t)tab.size();
   for (int64_t t = 0; t < ((int64_t)1 << (b1 - b0)); ++t) {
     int64_t v = 0;
     for (int b = b0; b < b1; ++b)
-      if ((t >> (b - b0)) & 1) v += pos ? ((int64_t)1 << vals[b]) : (int64_t)vals[b];
+      if ((t >> (b - b0)) & 1) v = pos ? (v ^ lay[vals[b]]) : v + (int64_t)vals[b];
     tab.push_back((int32_t)v);
   }
   return off;
+}
+
+// GF(2) rank of the low `bits` bits of vs[0..n).
+int low_rank(const int32_t* vs, int n, int bits) {
+  int32_t w[16];
+  for (int i = 0; i < n; ++i) w[i] = vs[i] & ((1 << bits) - 1);
+  int r = 0;
+  for (int b = 0; b < bits && r < n; ++b) {
+    int piv = -1;
+    for (int i = r; i < n; ++i)
+      if ((w[i] >> b) & 1) {
+        piv = i;
+        break;
+      }
+    if (piv < 0) continue;
+    std::swap(w[r], w[piv]);
+    for (int i = 0; i < n; ++i)
+      if (i != r && ((w[i] >> b) & 1)) w[i] ^= w[r];
+    ++r;
+  }
+  return r;
+}
+
+// LDS layout of a fused-chain tile: tile bit p is stored at address bit
+// vector lay[p] (the address of an element is the XOR over its set bits),
+// with lay[p] = 2^p plus bits below min(p, 6), so the map is invertible and
+// stays inside the tile.  Every LDS access of the kernel has its 64 lanes
+// varying in six known tile bits (gather: the first six gathered bits;
+// scatter: the first six scattered; stage reads: four R bits and K bits 0-1;
+// stage writes: four R bits and M bits 2-3).  Lanes 0-15, 0-31 and 0-63 of
+// an access then hit distinct 8-byte bank slots exactly when the low 4, 5
+// and 6 address bits of those tile bits are linearly independent; a
+// coordinate search over the low bits of each lay[p] minimises the lane
+// collisions summed over all accesses.
+std::vector<int32_t> chain_layout(int tb, const std::vector<std::vector<int32_t>>& sets) {
+  std::vector<int32_t> lay(tb);
+  for (int p = 0; p < tb; ++p) lay[p] = 1 << p;
+  auto cost = [&]() {
+    int c = 0;
+    int32_t v[16];
+    for (const auto& s : sets)
+      for (int g = 4; g <= 6; ++g) {
+        if ((int)s.size() < g) continue;
+        for (int i = 0; i < g; ++i) v[i] = lay[s[i]];
+        c += (1 << (g - low_rank(v, g, g))) - 1;
+      }
+    return c;
+  };
+  int best = cost();
+  for (int round = 0; round < 4 && best > 0; ++round) {
+    bool better = false;
+    for (int p = 0; p < tb; ++p) {
+      const int lowbits = std::min(p, 6);
+      const int32_t keep = lay[p];
+      int32_t pick = keep;
+      for (int32_t l = 0; l < (1 << lowbits); ++l) {
+        lay[p] = (1 << p) | l;
+        const int c = cost();
+        if (c < best) {
+          best = c;
+          pick = lay[p];
+          better = true;
+        }
+      }
+      lay[p] = pick;
+    }
+    if (!better) break;
+  }
+  return lay;
 }
 
 // Reads and checks the program of fused-chain step i (format: PQ_CHAIN_* in
@@ -271,14 +342,7 @@ const char* parse_chain(pq_ctx* c, pq_plan* p, const pq_plan_desc* d, int i, PSt
   a.ihi = nin - a.ilo;
   a.olo = std::min(nout, CH_LO_BITS);
   a.ohi = nout - a.olo;
-  a.ilp = bit_table(tab, ipos, 0, a.ilo, true);
-  a.ilg = bit_table(tab, istr, 0, a.ilo, false);
-  a.ihp = bit_table(tab, ipos, a.ilo, nin, true);
-  a.ihg = bit_table(tab, istr, a.ilo, nin, false);
-  a.olp = bit_table(tab, opos, 0, a.olo, true);
-  a.olg = bit_table(tab, ostr, 0, a.olo, false);
-  a.ohp = bit_table(tab, opos, a.olo, nout, true);
-  a.ohg = bit_table(tab, ostr, a.olo, nout, false);
+  std::vector<std::vector<int32_t>> kps(L), kss(L), mps(L), mss(L), rps(L);
   st.ops.clear();
   for (int t = 0; t < L; ++t) {
     ChainStage& g = a.st[t];
@@ -306,24 +370,57 @@ const char* parse_chain(pq_ctx* c, pq_plan* p, const pq_plan_desc* d, int i, PSt
     g.nm = nm;
     g.rlo = std::min(nr, 4);
     g.rhi = nr - g.rlo;
-    g.kp = bit_table(tab, kpos, 0, nk, true);
-    g.sk = bit_table(tab, kstr, 0, nk, false);
-    g.mp = bit_table(tab, mpos, 0, nm, true);
-    g.sm = bit_table(tab, mstr, 0, nm, false);
-    g.rl = bit_table(tab, rpos, 0, g.rlo, true);
-    g.rh = bit_table(tab, rpos, g.rlo, nr, true);
+    kps[t] = kpos;
+    kss[t] = kstr;
+    mps[t] = mpos;
+    mss[t] = mstr;
+    rps[t] = rpos;
     g.sd = a.sdn;
     a.sdn += 1 << (nk + nm);
     g.ko = a.idn;
     g.mo = a.idn + (1 << nk);
     g.ro = a.idn + (1 << nk) + (1 << nm);
-    a.idn += (1 << nk) + (1 << nm) + (1 << g.rlo);
+    g.rho = g.ro + (1 << g.rlo);
+    a.idn += (1 << nk) + (1 << nm) + (1 << g.rlo) + (1 << g.rhi);
     if ((int64_t)a.sdn * 8 + (int64_t)a.idn * 4 > CH_STAGE_BYTES) return "fused-chain operators exceed the LDS staging area";
     st.ops.push_back(os);
     st.level = std::max(st.level, 1 + (S.leaf ? 0 : p->steps[S.producer].level));
     p->slots[os].last_use = i;
     st.flops += 8.0 * (double)((int64_t)1 << (nr + nk + nm)) * (double)a.spec.total;
     st.bytes += 8.0 * (double)S.elems;
+  }
+  // tile layout for the accesses' lane bits, then the address tables
+  std::vector<std::vector<int32_t>> sets;
+  sets.push_back(std::vector<int32_t>(ipos.begin(), ipos.begin() + std::min(nin, 6)));
+  sets.push_back(std::vector<int32_t>(opos.begin(), opos.begin() + std::min(nout, 6)));
+  for (int t = 0; t < L; ++t) {
+    const int rl = a.st[t].rlo;
+    std::vector<int32_t> rd(rps[t].begin(), rps[t].begin() + rl), wr = rd;
+    if (rl == 4) {
+      for (int b = 0; b < std::min<int>(2, (int)kps[t].size()); ++b) rd.push_back(kps[t][b]);
+      for (int b = 2; b < std::min<int>(4, (int)mps[t].size()); ++b) wr.push_back(mps[t][b]);
+    }
+    sets.push_back(rd);
+    sets.push_back(wr);
+  }
+  const std::vector<int32_t> lay = chain_layout(tb, sets);
+  a.ilp = bit_table(tab, ipos, 0, a.ilo, true, lay);
+  a.ilg = bit_table(tab, istr, 0, a.ilo, false, lay);
+  a.ihp = bit_table(tab, ipos, a.ilo, nin, true, lay);
+  a.ihg = bit_table(tab, istr, a.ilo, nin, false, lay);
+  a.olp = bit_table(tab, opos, 0, a.olo, true, lay);
+  a.olg = bit_table(tab, ostr, 0, a.olo, false, lay);
+  a.ohp = bit_table(tab, opos, a.olo, nout, true, lay);
+  a.ohg = bit_table(tab, ostr, a.olo, nout, false, lay);
+  for (int t = 0; t < L; ++t) {
+    ChainStage& g = a.st[t];
+    const int nk = g.nk, nm = g.nm, nr = g.rlo + g.rhi;
+    g.kp = bit_table(tab, kps[t], 0, nk, true, lay);
+    g.sk = bit_table(tab, kss[t], 0, nk, false, lay);
+    g.mp = bit_table(tab, mps[t], 0, nm, true, lay);
+    g.sm = bit_table(tab, mss[t], 0, nm, false, lay);
+    g.rl = bit_table(tab, rps[t], 0, g.rlo, true, lay);
+    g.rh = bit_table(tab, rps[t], g.rlo, nr, true, lay);
   }
   st.bytes += 8.0 * (double)a.spec.total * (double)(((int64_t)1 << nin) + ((int64_t)1 << nout));
   st.spec.kc = PQ_KC_CHAIN;

@@ -379,7 +379,8 @@ def chain_program(slots, steps, group, tile_bits=CHAIN_TILE_BITS):
     if top > tile_bits:
         return None
     # operators and per-stage offsets staged in LDS beside the tile
-    staged = sum(8 * (1 << (len(kp) + len(mp))) + 4 * ((1 << len(kp)) + (1 << len(mp)) + (1 << min(len(rb), 4)))
+    staged = sum(8 * (1 << (len(kp) + len(mp)))
+                 + 4 * ((1 << len(kp)) + (1 << len(mp)) + (1 << min(len(rb), 4)) + (1 << (len(rb) - min(len(rb), 4))))
                  for _, kp, mp, rb in words_st)
     if staged > _lib.PQ_CHAIN_STAGE_BYTES:
         return None
