@@ -30,6 +30,13 @@ namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
+// A complex64 value in LDS: 8-byte aligned, so each lane moves it with one
+// ds_read_b64 / ds_write_b64 (a 4-byte aligned pair becomes two 32-bit
+// accesses, which put 32 lanes on only 16 of the banks).
+struct __attribute__((aligned(8))) lc {
+  float x, y;
+};
+
 // LDS addresses of tile elements come from the host's tables (pq_plan.hip,
 // chain_layout): a linear map over GF(2) chosen per chain so that the lanes
 // of every access below fall on distinct banks; addresses of disjoint bit
@@ -47,14 +54,14 @@ __device__ __forceinline__ floatx4 mfma(float a, float b, floatx4 c) {
 // are at most 8 chunk-blocks of them; the 32 x 32 case rereads them from
 // LDS per M block (register room for the next tile's prefetch).
 template <int NKC, int NMB>
-__device__ __forceinline__ void stage(c64* tile, const c64* sd, const int* si_, const ChainStage& g, int wave, int li,
+__device__ __forceinline__ void stage(lc* tile, const lc* sd, const int* si_, const ChainStage& g, int wave, int li,
                                       int lg) {
   constexpr bool HOLD = NKC * NMB <= 8;
   constexpr int HMB = HOLD ? NMB : 1;
   const int nk = 1 << g.nk, nm = 1 << g.nm;
-  auto opval = [&](int kc, int mb) -> c64 {
+  auto opval = [&](int kc, int mb) -> lc {
     const int k = 4 * kc + lg, m = 16 * mb + li;
-    c64 v;
+    lc v;
     v.x = 0.f;
     v.y = 0.f;
     if (k < nk && m < nm) v = sd[g.sd + k * nm + m];
@@ -69,7 +76,7 @@ __device__ __forceinline__ void stage(c64* tile, const c64* sd, const int* si_, 
     if constexpr (HOLD) {
 #pragma unroll
       for (int mb = 0; mb < NMB; ++mb) {
-        const c64 v = opval(kc, mb);
+        const lc v = opval(kc, mb);
         sr[kc][mb] = v.x;
         si[kc][mb] = v.y;
       }
@@ -90,47 +97,64 @@ __device__ __forceinline__ void stage(c64* tile, const c64* sd, const int* si_, 
   const bool full = (1 << g.rlo) == 16 && nm >= 16 * NMB;
   const int nrb = 1 << g.rhi;
   constexpr int NW = CH_THREADS / 64;
-  for (int rb = wave; rb < nrb; rb += NW) {
-    const int roff = rlo ^ si_[g.rho + rb];
-    // every K value of these r's is read before any M value is written
-    float br[NKC], bi[NKC];
+  // UNR blocks of 16 r's per iteration: the K values of all of them are
+  // read first, so a wave pays the LDS latency once per iteration; then each
+  // block's MFMAs run and its M values are written (blocks hold different
+  // r's, so a block may be written before a later one is read).  A block
+  // number past the end repeats the wave's first block and is not written.
+  constexpr int UNR = (NKC <= 4 && NMB == 1) ? 4 : (NKC * NMB <= 4 ? 2 : 1);
+  for (int rb0 = wave * UNR; rb0 < nrb; rb0 += NW * UNR) {
+    int roff[UNR];
+    float br[UNR][NKC], bi[UNR][NKC];
 #pragma unroll
-    for (int kc = 0; kc < NKC; ++kc) {
-      c64 v = tile[roff ^ kof[kc]];
-      if (kpart && lg >= nk) {
-        v.x = 0.f;
-        v.y = 0.f;
-      }
-      br[kc] = v.x;
-      bi[kc] = v.y;
-    }
-#pragma unroll
-    for (int mb = 0; mb < NMB; ++mb) {
-      if constexpr (!HOLD) {
-#pragma unroll
-        for (int kc = 0; kc < NKC; ++kc) {
-          const c64 v = opval(kc, mb);
-          sr[kc][0] = v.x;
-          si[kc][0] = v.y;
-        }
-      }
-      constexpr int hm = 0;
-      floatx4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+    for (int u = 0; u < UNR; ++u) {
+      const int rb = rb0 + u < nrb ? rb0 + u : rb0;
+      roff[u] = rlo ^ si_[g.rho + rb];
 #pragma unroll
       for (int kc = 0; kc < NKC; ++kc) {
-        const float ar = sr[kc][HOLD ? mb : hm], ai = si[kc][HOLD ? mb : hm];
-        cr = mfma(ar, br[kc], cr);
-        ci = mfma(ar, bi[kc], ci);
-        cr = mfma(-ai, bi[kc], cr);
-        ci = mfma(ai, br[kc], ci);
+        lc v = tile[roff[u] ^ kof[kc]];
+        if (kpart && lg >= nk) {
+          v.x = 0.f;
+          v.y = 0.f;
+        }
+        br[u][kc] = v.x;
+        bi[u][kc] = v.y;
       }
+    }
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        if (full || (rv && mof[mb][i] >= 0)) {
-          c64 v;
-          v.x = cr[i];
-          v.y = ci[i];
-          tile[roff ^ mof[mb][i]] = v;
+    for (int u = 0; u < UNR; ++u) {
+      const bool wr = UNR == 1 || rb0 + u < nrb;
+#pragma unroll
+      for (int mb = 0; mb < NMB; ++mb) {
+        if constexpr (!HOLD) {
+#pragma unroll
+          for (int kc = 0; kc < NKC; ++kc) {
+            const lc v = opval(kc, mb);
+            sr[kc][0] = v.x;
+            si[kc][0] = v.y;
+          }
+        }
+        constexpr int hm = 0;
+        floatx4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < NKC; ++kc) {
+          const float ar = sr[kc][HOLD ? mb : hm], ai = si[kc][HOLD ? mb : hm];
+          cr = mfma(ar, br[u][kc], cr);
+          ci = mfma(ar, bi[u][kc], ci);
+          cr = mfma(-ai, bi[u][kc], cr);
+          ci = mfma(ai, br[u][kc], ci);
+        }
+        // accumulator register i holds row 4 lg + i, column li
+        if (wr) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            if (full || (rv && mof[mb][i] >= 0)) {
+              lc v;
+              v.x = cr[i];
+              v.y = ci[i];
+              tile[roff[u] ^ mof[mb][i]] = v;
+            }
+          }
         }
       }
     }
@@ -138,25 +162,26 @@ __device__ __forceinline__ void stage(c64* tile, const c64* sd, const int* si_, 
 }
 
 __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
-  extern __shared__ c64 smem[];
+  extern __shared__ lc smem[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int li = lane & 15, lg = lane >> 4;
   const int32_t* tab = a.tab;
   const uint32_t total = a.spec.total, G = gridDim.x;
-  c64* tile = smem;
+  lc* tile = smem;
 
   // stage every operator (dense [k][m]: the 16 lanes of a lane group read
   // 16 consecutive m) and the K / M / R tile addresses in LDS after the
   // tile, once per workgroup
-  c64* sd = smem + (1 << a.tb);
+  lc* sd = smem + (1 << a.tb);
   int* sidx = reinterpret_cast<int*>(sd + a.sdn);
   for (int s = 0; s < a.nst; ++s) {
     const ChainStage& g = a.st[s];
     const int nk = 1 << g.nk, nkm = 1 << (g.nk + g.nm);
     for (int e = tid; e < nkm; e += CH_THREADS) {
       const int k = e >> g.nm, m = e & ((1 << g.nm) - 1);
-      sd[g.sd + e] = g.S[tab[g.sk + k] + tab[g.sm + m]];
+      const c64 v = g.S[tab[g.sk + k] + tab[g.sm + m]];
+      sd[g.sd + e] = lc{v.x, v.y};
     }
     for (int e = tid; e < nk; e += CH_THREADS) sidx[g.ko + e] = tab[g.kp + e];
     for (int e = tid; e < (1 << g.nm); e += CH_THREADS) sidx[g.mo + e] = tab[g.mp + e];
@@ -190,15 +215,15 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
 #pragma unroll
     for (int it = 0; it < CH_NIT; ++it) P[it] = src[o[it]];
   };
-  auto put = [&](c64* buf) {
+  auto put = [&](lc* buf) {
 #pragma unroll
-    for (int it = 0; it < CH_NIT; ++it) buf[gp0 ^ hid[it]] = P[it];
+    for (int it = 0; it < CH_NIT; ++it) buf[gp0 ^ hid[it]] = lc{P[it].x, P[it].y};
   };
-  auto scatter = [&](const c64* buf, uint32_t t) {
+  auto scatter = [&](const lc* buf, uint32_t t) {
     int32_t ox, oy;
     group_offsets(a.spec, t, ox, oy);
     c64* dst = a.Y + oy + sg0;
-    c64 v[CH_NIT];
+    lc v[CH_NIT];
     int o[CH_NIT];
 #pragma unroll
     for (int it = 0; it < CH_NIT; ++it) {
@@ -206,7 +231,12 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
       o[it] = hid[3 * CH_NIT + it];
     }
 #pragma unroll
-    for (int it = 0; it < CH_NIT; ++it) dst[o[it]] = v[it];
+    for (int it = 0; it < CH_NIT; ++it) {
+      c64 w;
+      w.x = v[it].x;
+      w.y = v[it].y;
+      dst[o[it]] = w;
+    }
   };
 
   uint32_t t = blockIdx.x;
@@ -251,7 +281,7 @@ hipError_t launch_chain(const ChainArgs& a, hipStream_t s) {
   if (a.ilo > CH_LO_BITS || a.olo > CH_LO_BITS || a.ihi > CH_TB - CH_LO_BITS || a.ohi > CH_TB - CH_LO_BITS)
     return hipErrorInvalidValue;
   if ((int64_t)a.sdn * 8 + (int64_t)a.idn * 4 > CH_STAGE_BYTES) return hipErrorInvalidValue;
-  const size_t lds = (sizeof(c64) << a.tb) + sizeof(c64) * a.sdn + sizeof(int) * (a.idn + 4 * CH_NIT);
+  const size_t lds = (sizeof(lc) << a.tb) + sizeof(lc) * a.sdn + sizeof(int) * (a.idn + 4 * CH_NIT);
   // persistent workgroups: as many as fit on the device at once with this
   // LDS size (two per CU for a 2^13-element tile)
   static int ncu[64] = {0};

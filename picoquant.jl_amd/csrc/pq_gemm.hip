@@ -697,9 +697,43 @@ hipError_t launch_tile_h(const c64* A, const c64* B, c64* Out, const GemmArgs& g
   return hipGetLastError();
 }
 
+template <typename C>
+__global__ __launch_bounds__(256) void k_sum_partials(const C* P, C* Out, const Group gm, const Group gn,
+                                                      uint32_t mt, uint64_t mn, int parts) {
+  const uint64_t o = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (o >= mn) return;
+  int32_t pm, cm, pn, cn;
+  group_offsets(gm, (uint32_t)(o % mt), pm, cm);
+  group_offsets(gn, (uint32_t)(o / mt), pn, cn);
+  double re = 0.0, im = 0.0;
+  for (int z = 0; z < parts; ++z) {
+    const C v = P[(uint64_t)z * mn + o];
+    re += (double)v.x;
+    im += (double)v.y;
+  }
+  C r;
+  r.x = (decltype(r.x))re;
+  r.y = (decltype(r.y))im;
+  Out[(int64_t)cm + cn] = r;
+}
+
 }  // namespace
 
 int gemm_k_limit() { return (int)GEMM_K_MAX; }
+
+template <typename C>
+hipError_t launch_sum_partials(const C* P, C* Out, const Group& gm, const Group& gn, uint32_t mt, uint64_t mn,
+                               int parts, hipStream_t s) {
+  if (mt < 1 || mn < 1 || parts < 1 || mn % mt != 0 || (uint64_t)gm.total != mt || (uint64_t)gn.total != mn / mt)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_sum_partials<C>), dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s, P, Out, gm, gn, mt,
+                     mn, parts);
+  return hipGetLastError();
+}
+template hipError_t launch_sum_partials<c128>(const c128*, c128*, const Group&, const Group&, uint32_t, uint64_t,
+                                              int, hipStream_t);
+template hipError_t launch_sum_partials<c64>(const c64*, c64*, const Group&, const Group&, uint32_t, uint64_t, int,
+                                             hipStream_t);
 
 template <typename C>
 hipError_t launch_gemm(const C* A, const C* B, C* Out, const GemmArgs& g, hipStream_t s) {

@@ -72,6 +72,12 @@ template <typename C>
 hipError_t launch_copy(const C* In, C* Out, const CopyArgs& a, hipStream_t s);
 template <typename C>
 hipError_t launch_gemm(const C* A, const C* B, C* Out, const GemmArgs& a, hipStream_t s);
+// Split-K second pass: Out[C offset of (m, n)] = sum over z < parts of
+// P[z * mn + m + mt * n], summed in FP64; gm / gn give each output's C
+// offset (s1) for its row m < mt and column n (s0 unused).
+template <typename C>
+hipError_t launch_sum_partials(const C* P, C* Out, const Group& gm, const Group& gn, uint32_t mt, uint64_t mn,
+                               int parts, hipStream_t s);
 int gemm_k_limit();
 // complex64 GEMM on FP16 hi/lo matrix cores (contiguous K, multiple of 32).
 bool gemm16_applicable(const GemmArgs& g);

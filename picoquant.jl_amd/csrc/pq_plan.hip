@@ -698,6 +698,27 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
           !inside(B, offB, span_of(st.spec.N, true) + span_of(st.spec.K, false)) ||
           !inside(Cs, offC, span_of(st.spec.M, false) + span_of(st.spec.N, false)))
         return bad("batch entries out of range");
+      // A batch mode that one operand does not depend on is a free mode of
+      // the other side (the shared operand is then read once, in one larger
+      // product); modes that both operands or neither depend on stay batch
+      // modes.  The offsets read and written are the same.
+      Batch keep = single_batch();
+      keep.nmodes = 0;
+      for (int q = 0; q < L.nmodes; ++q) {
+        if (L.sB[q] == 0 && L.sA[q] != 0 && (int)st.spec.M.size() < PQ_MAX_MODES) {
+          st.spec.M.push_back({2, L.sA[q], L.sC[q]});
+        } else if (L.sA[q] == 0 && L.sB[q] != 0 && (int)st.spec.N.size() < PQ_MAX_MODES) {
+          st.spec.N.push_back({2, L.sB[q], L.sC[q]});
+        } else {
+          const int k = keep.nmodes++;
+          keep.ext[k] = L.ext[q];
+          keep.sA[k] = L.sA[q];
+          keep.sB[k] = L.sB[q];
+          keep.sC[k] = L.sC[q];
+          keep.total *= 2;
+        }
+      }
+      L = keep;
     }
     st.spec.arith = d->step_arith ? d->step_arith[i] : 0;
     if (st.spec.arith < 0 || st.spec.arith > 1) return bad("bad step arithmetic");
@@ -709,7 +730,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
       free_plan(p);
       return fail(c, r, "step " + std::to_string(i) + ": " + msg);
     }
-    const double nbat = (double)Cs.copies() * (st.phase2 ? (double)p->n_groups : 1.0);
+    const double nbat = (double)st.spec.L.total * (st.phase2 ? (double)p->n_groups : 1.0);
     const double es = (double)esize_of(st.spec.dtype);
     st.flops = 8.0 * st.spec.Mt * st.spec.Nt * st.spec.Kt * nbat;
     st.bytes = es * (st.spec.Mt * st.spec.Kt + st.spec.Kt * st.spec.Nt + st.spec.Mt * st.spec.Nt) * nbat;

@@ -49,6 +49,7 @@ struct StepSpec {
   bool gemm_swap = false; // GEMM runs as C^T = B^T A^T (C's unit stride on the M side)
   int arith = -1;         // complex64 GEMM: 0 FP32, 1 FP16 hi/lo; -1 = the context's
   int dtype = -1;         // PQ_C128 / PQ_C64; -1 = the context's
+  int splitk = 1;         // GEMM: K cut into this many partial products in scratch, then summed
   uint64_t Mt = 1, Nt = 1, Kt = 1;
 };
 

@@ -215,6 +215,37 @@ static int pick_tile(uint64_t Mt, uint64_t Nt, size_t esize) {
   return 3;
 }
 
+// Workgroup tile (rows, columns) of GEMM tile index t, as launch_gemm runs it.
+static void gemm_tile_dims(int t, uint64_t& bm, uint64_t& bn) {
+  static const int d[PQ_GEMM_TILES][2] = {{32, 32},  {64, 32},  {32, 64},  {64, 64},   {128, 64},
+                                          {64, 128}, {64, 16},  {128, 16}, {256, 16},  {16, 64},
+                                          {128, 32}, {128, 128}, {16, 128}, {16, 256}, {32, 128}};
+  bm = (uint64_t)d[t][0];
+  bn = (uint64_t)d[t][1];
+}
+
+// Split-K: a GEMM whose output tiles cannot fill the device (a long K with
+// few rows and columns) runs as splitk partial products over consecutive
+// parts of its slowest K mode, written to scratch and summed in FP64 by
+// k_sum_partials.  Parts are halved until about GEMM_SPLIT_WGS workgroups
+// run, each part keeps at least GEMM_SPLIT_MIN_K values of K, and at most
+// GEMM_SPLIT_MAX parts are made.
+#define GEMM_SPLIT_WGS 1024
+#define GEMM_SPLIT_MIN_K 512
+#define GEMM_SPLIT_MAX 16
+static int gemm_split(const StepSpec& s, uint64_t m, uint64_t n) {
+  if (s.L.total != 1 || s.K.empty() || s.L.nmodes >= PQ_MAX_BMODES) return 1;
+  uint64_t bm, bn;
+  gemm_tile_dims(s.tile, bm, bn);
+  const uint64_t tiles = ((m + bm - 1) / bm) * ((n + bn - 1) / bn);
+  const uint64_t last = (uint64_t)s.K.back().ext;
+  int parts = 1;
+  while (tiles * parts < GEMM_SPLIT_WGS && parts < GEMM_SPLIT_MAX && last % (2 * parts) == 0 &&
+         s.Kt / (2 * parts) >= GEMM_SPLIT_MIN_K)
+    parts *= 2;
+  return parts;
+}
+
 static bool stream_fits(uint64_t Kt, uint64_t Ny, size_t esize) {
   uint64_t KP = 1;
   while (KP < Kt) KP <<= 1;
@@ -271,6 +302,7 @@ int resolve_step(pq_ctx* c, StepSpec& s, size_t esize) {
       s.tile = n <= 16 ? 6 : (m <= 16 ? 9 : 3);
     else
       s.tile = pick_tile(m, n, esize);
+    s.splitk = kc == PQ_KC_GEMM ? gemm_split(s, m, n) : 1;
   } else if (kc == PQ_KC_REDUCE) {
     if (Mt * Nt > 16) return fail(c, PQ_ERR_UNSUPPORTED, "reduce kernel needs M*N <= 16");
   } else if (kc != PQ_KC_GENERIC) {
@@ -282,6 +314,7 @@ int resolve_step(pq_ctx* c, StepSpec& s, size_t esize) {
 }
 
 size_t step_scratch_bytes(const StepSpec& s, size_t esize) {
+  if (s.kc == PQ_KC_GEMM && s.splitk > 1) return (size_t)s.splitk * s.Mt * s.Nt * esize;
   if (s.kc != PQ_KC_REDUCE) return 0;
   return (size_t)reduce_blocks_for((uint32_t)s.Kt) * s.Mt * s.Nt * s.L.total * esize;
 }
@@ -350,7 +383,50 @@ static int launch_typed(pq_ctx* c, const C* pa, const C* pb, C* pc, const StepSp
     }
     bool split = false;
     if constexpr (sizeof(C) == 8) split = s.kc == PQ_KC_GEMM16 && gemm16_applicable(ga);
-    if (split) {
+    if (s.kc == PQ_KC_GEMM && s.splitk > 1) {
+      // partial products: the slowest K mode is cut into splitk parts (a
+      // batch mode); part z writes a compact [m][n] block of scratch
+      if (!scratch || L.total != 1 || gK.empty() || gK.back().ext % s.splitk != 0)
+        return fail(c, PQ_ERR_UNSUPPORTED, "split-K needs scratch and an unbatched step");
+      const int parts = s.splitk;
+      std::vector<Mode> pK = gK;
+      Mode& km = pK.back();
+      km.ext /= parts;
+      const int64_t mt = (int64_t)(s.gemm_swap ? s.Nt : s.Mt), mn = (int64_t)(s.Mt * s.Nt);
+      std::vector<Mode> pM = gM, pN = gN, rM, rN;
+      int64_t run = 1;
+      for (auto& m : pM) {
+        rM.push_back({m.ext, run, m.s1});   // partial offset, C offset
+        m.s1 = run;
+        run *= m.ext;
+      }
+      run = mt;
+      for (auto& m : pN) {
+        rN.push_back({m.ext, run, m.s1});
+        m.s1 = run;
+        run *= m.ext;
+      }
+      GemmArgs gp = ga;
+      if (!fill_group(gp.M, pM) || !fill_group(gp.N, pN) || !fill_group(gp.K, pK))
+        return fail(c, PQ_ERR_UNSUPPORTED, "too many modes");
+      gp.L = single_batch();
+      gp.L.nmodes = 1;
+      gp.L.ext[0] = make_fdiv((uint32_t)parts);
+      gp.L.sA[0] = (int64_t)km.ext * km.s0;
+      gp.L.sB[0] = (int64_t)km.ext * km.s1;
+      gp.L.sC[0] = mn;
+      gp.L.total = (uint32_t)parts;
+      {
+        const int64_t vec = 16 / (int64_t)sizeof(C);
+        if (gp.contig && (gp.L.sA[0] % vec || gp.L.sB[0] % vec)) gp.contig = 0;
+      }
+      Group sm, sn;
+      if (!fill_group(sm, rM) || !fill_group(sn, rN)) return fail(c, PQ_ERR_UNSUPPORTED, "too many modes");
+      ProfScope ps(c, PQ_KC_GEMM, alg_bytes, alg_flops);
+      C* part = reinterpret_cast<C*>(scratch);
+      e = launch_gemm<C>(xa, xb, part, gp, c->stream);
+      if (e == hipSuccess) e = launch_sum_partials<C>(part, pc, sm, sn, (uint32_t)mt, (uint64_t)mn, parts, c->stream);
+    } else if (split) {
       ProfScope ps(c, PQ_KC_GEMM16, alg_bytes, alg_flops);
       if constexpr (sizeof(C) == 8) e = launch_gemm16(xa, xb, pc, ga, c->stream);
     } else {

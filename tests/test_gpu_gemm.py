@@ -131,6 +131,29 @@ def test_long_k_beyond_one_offset_table(ctx, kfast):
         ctx.free(x)
 
 
+@pytest.mark.parametrize("orient", [0, 1], ids=["as-given", "transposed"])
+@pytest.mark.parametrize("M,N,K", [(32, 64, 16384), (64, 16, 8192), (40, 24, 6144)])
+def test_split_k_partial_products(ctx, M, N, K, orient):
+    """Output tiles far fewer than the device's CUs with K >= 6144: the GEMM
+    runs as K-parts written to scratch and summed in FP64 (at least 4 parts
+    for each shape here: M*N spans at most a few 32x32 tiles, and every part
+    keeps >= 512 of K).  Contiguous and strided K, ragged M/N, both launch
+    orientations, against the oracle; then the same split K through two K
+    modes (the slow one cut) and through the automatic dispatch."""
+    rng = np.random.default_rng(M + N + K)
+    A = rand(rng, (K, M))
+    B = rand(rng, (N, K))
+    check(ctx, A, [1, -1], B, [-2, 1], gemm(ctx, A, [1, -1], B, [-2, 1], orient=orient))
+    A2 = rand(rng, (M, K // 64, 64))
+    B2 = rand(rng, (64, N, K // 64))
+    check(ctx, A2, [-1, 2, 1], B2, [1, -2, 2], gemm(ctx, A2, [-1, 2, 1], B2, [1, -2, 2], orient=orient))
+    ha, hb = ctx.upload(A), ctx.upload(B)
+    hc = ctx.contract(ha, [1, -1], hb, [-2, 1])
+    check(ctx, A, [1, -1], B, [-2, 1], ctx.download(hc))
+    for x in (ha, hb, hc):
+        ctx.free(x)
+
+
 GEMM16_TOL = 1e-5    # relative to sum |a||b|: hi/lo split drops lo*lo (~2^-22) and rounds lo
 
 
