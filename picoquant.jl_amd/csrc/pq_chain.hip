@@ -244,11 +244,13 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
     load(t);
     put(tile);
   }
-  if (t + G < total) load(t + G);
   __syncthreads();
 
   // every wave of a workgroup runs the same iterations, and the loop ends
-  // when the workgroup's tiles run out
+  // when the workgroup's tiles run out.  The next tile's loads are issued
+  // after the first stage (the memory pipeline is idle during the stages,
+  // and the previous tile's stores have drained by then), its data is put
+  // into the tile after this tile's scatter.
   for (; t < total; t += G) {
     for (int s = 0; s < a.nst; ++s) {
       const ChainStage& g = a.st[s];
@@ -263,13 +265,12 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
         case 4 * 4 + 2: stage<4, 2>(tile, sd, sidx, g, wave, li, lg); break;
         default: stage<8, 2>(tile, sd, sidx, g, wave, li, lg); break;
       }
+      if (s == 0 && t + G < total) load(t + G);
       __syncthreads();
     }
     scatter(tile, t);
     __syncthreads();
-    // the next tile's loads were issued before this tile's stages
     if (t + G < total) put(tile);
-    if (t + 2 * G < total) load(t + 2 * G);
     __syncthreads();
   }
 }
