@@ -188,11 +188,12 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
     for (int e = tid; e < (1 << g.rlo); e += CH_THREADS) sidx[g.ro + e] = tab[g.rl + e];
     for (int e = tid; e < (1 << g.rhi); e += CH_THREADS) sidx[g.rho + e] = tab[g.rh + e];
   }
-  // Gather / scatter without branches: element j = lane + 2^lo it of the
-  // tile; each lane does CH_NIT iterations, and the staged iteration
-  // tables wrap around (entry it holds iteration it mod 2^hi), as do the
-  // lanes beyond 2^lo.  A repeated iteration rereads the same element into
-  // the same tile position, or stores the same value to the same address.
+  // Gather / scatter without branches: lane l, iteration it moves the
+  // element pair (l + 2^lo it) along the side's pair bit; each lane does
+  // CH_NIT iterations, and the staged iteration tables wrap around (entry
+  // it holds iteration it mod 2^hi), as do the lanes beyond 2^lo.  A
+  // repeated iteration rereads the same elements into the same tile
+  // positions, or stores the same values to the same addresses.
   int* hid = sidx + a.idn;   // tile address / tensor offset per iteration
   if (tid < CH_NIT) {
     hid[tid] = tab[a.ihp + (tid & ((1 << a.ihi) - 1))];
@@ -204,7 +205,7 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
   const int gt = tid & ((1 << a.ilo) - 1), st = tid & ((1 << a.olo) - 1);
   const int gp0 = tab[a.ilp + gt], sp0 = tab[a.olp + st];
   const int32_t gg0 = tab[a.ilg + gt], sg0 = tab[a.olg + st];
-  c64 P[CH_NIT];   // the next tile, in flight
+  c64 P[2 * CH_NIT];   // the next tile, in flight (pairs)
   auto load = [&](uint32_t t) {
     int32_t ox, oy;
     group_offsets(a.spec, t, ox, oy);
@@ -212,30 +213,65 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
     int o[CH_NIT];
 #pragma unroll
     for (int it = 0; it < CH_NIT; ++it) o[it] = hid[CH_NIT + it];
+    if (a.gvec) {
 #pragma unroll
-    for (int it = 0; it < CH_NIT; ++it) P[it] = src[o[it]];
+      for (int it = 0; it < CH_NIT; ++it) {
+        const float4 v = *reinterpret_cast<const float4*>(src + o[it]);
+        P[2 * it].x = v.x;
+        P[2 * it].y = v.y;
+        P[2 * it + 1].x = v.z;
+        P[2 * it + 1].y = v.w;
+      }
+    } else {
+#pragma unroll
+      for (int it = 0; it < CH_NIT; ++it) {
+        P[2 * it] = src[o[it]];
+        P[2 * it + 1] = src[o[it] + a.gps];
+      }
+    }
   };
   auto put = [&](lc* buf) {
 #pragma unroll
-    for (int it = 0; it < CH_NIT; ++it) buf[gp0 ^ hid[it]] = lc{P[it].x, P[it].y};
+    for (int it = 0; it < CH_NIT; ++it) {
+      const int q = gp0 ^ hid[it];
+      buf[q] = lc{P[2 * it].x, P[2 * it].y};
+      buf[q ^ a.gpa] = lc{P[2 * it + 1].x, P[2 * it + 1].y};
+    }
   };
   auto scatter = [&](const lc* buf, uint32_t t) {
     int32_t ox, oy;
     group_offsets(a.spec, t, ox, oy);
     c64* dst = a.Y + oy + sg0;
-    lc v[CH_NIT];
+    lc v[2 * CH_NIT];
     int o[CH_NIT];
 #pragma unroll
     for (int it = 0; it < CH_NIT; ++it) {
-      v[it] = buf[sp0 ^ hid[2 * CH_NIT + it]];
+      const int q = sp0 ^ hid[2 * CH_NIT + it];
+      v[2 * it] = buf[q];
+      v[2 * it + 1] = buf[q ^ a.spa];
       o[it] = hid[3 * CH_NIT + it];
     }
+    if (a.svec) {
 #pragma unroll
-    for (int it = 0; it < CH_NIT; ++it) {
-      c64 w;
-      w.x = v[it].x;
-      w.y = v[it].y;
-      dst[o[it]] = w;
+      for (int it = 0; it < CH_NIT; ++it) {
+        float4 w;
+        w.x = v[2 * it].x;
+        w.y = v[2 * it].y;
+        w.z = v[2 * it + 1].x;
+        w.w = v[2 * it + 1].y;
+        *reinterpret_cast<float4*>(dst + o[it]) = w;
+      }
+    } else {
+#pragma unroll
+      for (int it = 0; it < CH_NIT; ++it) {
+        c64 w0, w1;
+        w0.x = v[2 * it].x;
+        w0.y = v[2 * it].y;
+        w1.x = v[2 * it + 1].x;
+        w1.y = v[2 * it + 1].y;
+        dst[o[it]] = w0;
+        dst[o[it] + a.sps] = w1;
+      }
     }
   };
 
@@ -279,7 +315,10 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
 
 hipError_t launch_chain(const ChainArgs& a, hipStream_t s) {
   if (a.tb < 0 || a.tb > CH_TB || a.nst < 1 || a.nst > CH_ST || a.spec.total < 1) return hipErrorInvalidValue;
-  if (a.ilo > CH_LO_BITS || a.olo > CH_LO_BITS || a.ihi > CH_TB - CH_LO_BITS || a.ohi > CH_TB - CH_LO_BITS)
+  if (a.ilo > CH_LO_BITS || a.olo > CH_LO_BITS || a.ihi > CH_TB - 1 - CH_LO_BITS || a.ohi > CH_TB - 1 - CH_LO_BITS ||
+      a.ihi < 0 || a.ohi < 0)
+    return hipErrorInvalidValue;
+  if ((a.gvec && ((uintptr_t)a.X % 16 != 0 || !a.gvec_ok)) || (a.svec && ((uintptr_t)a.Y % 16 != 0 || !a.svec_ok)))
     return hipErrorInvalidValue;
   if ((int64_t)a.sdn * 8 + (int64_t)a.idn * 4 > CH_STAGE_BYTES) return hipErrorInvalidValue;
   const size_t lds = (sizeof(lc) << a.tb) + sizeof(lc) * a.sdn + sizeof(int) * (a.idn + 4 * CH_NIT);

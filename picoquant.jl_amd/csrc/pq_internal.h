@@ -98,7 +98,7 @@ int reduce_blocks_for(uint32_t ktotal);
 #define CH_ST 8    // PQ_CHAIN_MAX_STAGES
 #define CH_THREADS 512   // 8 waves per workgroup, two persistent workgroups per CU
 #define CH_LO_BITS 9     // gather / scatter bits taken from the lane (2^9 = CH_THREADS)
-#define CH_NIT (1 << (CH_TB - CH_LO_BITS))   // gather / scatter iterations per lane and tile
+#define CH_NIT (1 << (CH_TB - 1 - CH_LO_BITS))   // gather / scatter iterations (element pairs) per lane and tile
 struct ChainStage {
   const c64* S;
   int32_t nk, nm;          // 2^nk operator columns, 2^nm rows
@@ -117,6 +117,12 @@ struct ChainArgs {
   int32_t ilp, ilg, ihp, ihg;              // gather tables: low / high tile address, element offset
   int32_t olp, olg, ohp, ohg;              // scatter tables
   int32_t sdn, idn;                        // staged c64 operator elements, staged ints
+  // Each lane moves element pairs: the pair bit's tile address (XOR) and
+  // tensor stride on the gather and scatter side, and whether the pair is
+  // one aligned 16-byte access (pair stride 1, every other offset even)
+  int32_t gpa, gps, spa, sps;
+  int32_t gvec_ok, svec_ok;                // strides allow it (host); the launch also checks alignment
+  int32_t gvec, svec;
   Group spec;   // spectator modes: s0 = stride in X, s1 = stride in Y
   ChainStage st[CH_ST];
 };

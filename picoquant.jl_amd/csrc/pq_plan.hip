@@ -338,10 +338,45 @@ const char* parse_chain(pq_ctx* c, pq_plan* p, const pq_plan_desc* d, int i, PSt
   if (!slot_holds(c, X, xspan)) return "gather out of range";
   if (!slot_holds(c, Y, yspan)) return "scatter out of range";
   if (((int64_t)a.spec.total << nout) != Y.elems) return "scatter does not cover the output";
-  a.ilo = std::min(nin, CH_LO_BITS);
-  a.ihi = nin - a.ilo;
-  a.olo = std::min(nout, CH_LO_BITS);
-  a.ohi = nout - a.olo;
+  // Each side moves element pairs along one bit (the stride-1 bit if there
+  // is one, else the smallest stride; none for a side without bits: the
+  // pair is one element twice); the other bits go to the lane and
+  // iteration tables.
+  auto pair_bit = [&](const std::vector<int32_t>& str, int n) -> int {
+    int best = -1;
+    for (int bb = 0; bb < n; ++bb)
+      if (best < 0 || str[bb] < str[best]) best = bb;
+    return best;
+  };
+  const int gb = pair_bit(istr, nin), sb = pair_bit(ostr, nout);
+  auto even_rest = [&](const std::vector<int32_t>& str, int n, int pb, bool xside) -> bool {
+    if (pb < 0 || str[pb] != 1) return false;
+    for (int bb = 0; bb < n; ++bb)
+      if (bb != pb && (str[bb] & 1)) return false;
+    for (const Mode& md : spec)
+      if (((xside ? md.s0 : md.s1) & 1) && md.ext > 1) return false;
+    return true;
+  };
+  a.gvec_ok = even_rest(istr, nin, gb, true) ? 1 : 0;
+  a.svec_ok = even_rest(ostr, nout, sb, false) ? 1 : 0;
+  a.gps = gb >= 0 ? istr[gb] : 0;
+  a.sps = sb >= 0 ? ostr[sb] : 0;
+  std::vector<int32_t> ipos2, istr2, opos2, ostr2;
+  for (int bb = 0; bb < nin; ++bb)
+    if (bb != gb) {
+      ipos2.push_back(ipos[bb]);
+      istr2.push_back(istr[bb]);
+    }
+  for (int bb = 0; bb < nout; ++bb)
+    if (bb != sb) {
+      opos2.push_back(opos[bb]);
+      ostr2.push_back(ostr[bb]);
+    }
+  const int nin2 = (int)ipos2.size(), nout2 = (int)opos2.size();
+  a.ilo = std::min(nin2, CH_LO_BITS);
+  a.ihi = nin2 - a.ilo;
+  a.olo = std::min(nout2, CH_LO_BITS);
+  a.ohi = nout2 - a.olo;
   std::vector<std::vector<int32_t>> kps(L), kss(L), mps(L), mss(L), rps(L);
   st.ops.clear();
   for (int t = 0; t < L; ++t) {
@@ -391,8 +426,8 @@ const char* parse_chain(pq_ctx* c, pq_plan* p, const pq_plan_desc* d, int i, PSt
   }
   // tile layout for the accesses' lane bits, then the address tables
   std::vector<std::vector<int32_t>> sets;
-  sets.push_back(std::vector<int32_t>(ipos.begin(), ipos.begin() + std::min(nin, 6)));
-  sets.push_back(std::vector<int32_t>(opos.begin(), opos.begin() + std::min(nout, 6)));
+  sets.push_back(std::vector<int32_t>(ipos2.begin(), ipos2.begin() + std::min(nin2, 6)));
+  sets.push_back(std::vector<int32_t>(opos2.begin(), opos2.begin() + std::min(nout2, 6)));
   for (int t = 0; t < L; ++t) {
     const int rl = a.st[t].rlo;
     std::vector<int32_t> rd(rps[t].begin(), rps[t].begin() + rl), wr = rd;
@@ -404,14 +439,16 @@ const char* parse_chain(pq_ctx* c, pq_plan* p, const pq_plan_desc* d, int i, PSt
     sets.push_back(wr);
   }
   const std::vector<int32_t> lay = chain_layout(tb, sets);
-  a.ilp = bit_table(tab, ipos, 0, a.ilo, true, lay);
-  a.ilg = bit_table(tab, istr, 0, a.ilo, false, lay);
-  a.ihp = bit_table(tab, ipos, a.ilo, nin, true, lay);
-  a.ihg = bit_table(tab, istr, a.ilo, nin, false, lay);
-  a.olp = bit_table(tab, opos, 0, a.olo, true, lay);
-  a.olg = bit_table(tab, ostr, 0, a.olo, false, lay);
-  a.ohp = bit_table(tab, opos, a.olo, nout, true, lay);
-  a.ohg = bit_table(tab, ostr, a.olo, nout, false, lay);
+  a.gpa = gb >= 0 ? lay[ipos[gb]] : 0;
+  a.spa = sb >= 0 ? lay[opos[sb]] : 0;
+  a.ilp = bit_table(tab, ipos2, 0, a.ilo, true, lay);
+  a.ilg = bit_table(tab, istr2, 0, a.ilo, false, lay);
+  a.ihp = bit_table(tab, ipos2, a.ilo, nin2, true, lay);
+  a.ihg = bit_table(tab, istr2, a.ilo, nin2, false, lay);
+  a.olp = bit_table(tab, opos2, 0, a.olo, true, lay);
+  a.olg = bit_table(tab, ostr2, 0, a.olo, false, lay);
+  a.ohp = bit_table(tab, opos2, a.olo, nout2, true, lay);
+  a.ohg = bit_table(tab, ostr2, a.olo, nout2, false, lay);
   for (int t = 0; t < L; ++t) {
     ChainStage& g = a.st[t];
     const int nk = g.nk, nm = g.nm, nr = g.rlo + g.rhi;
@@ -468,6 +505,8 @@ int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
         a.X = (const c64*)slot_ptr(c, p, st.a, group);
         a.Y = (c64*)slot_ptr(c, p, st.c, group);
         for (int t = 0; t < a.nst; ++t) a.st[t].S = (const c64*)slot_ptr(c, p, st.ops[t], group);
+        a.gvec = a.gvec_ok && ((uintptr_t)a.X % 16) == 0;
+        a.svec = a.svec_ok && ((uintptr_t)a.Y % 16) == 0;
         ProfScope ps(c, PQ_KC_CHAIN, st.bytes, st.flops);
         hipError_t e = launch_chain(a, c->stream);
         if (e != hipSuccess) r = hipfail(c, e, "fused chain kernel");
