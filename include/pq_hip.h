@@ -130,6 +130,13 @@ int pq_set_arithmetic(pq_ctx* ctx, int mode);
  * that adjacent lanes store adjacent outputs), 0 = as given, 1 = transposed. */
 int pq_set_gemm_orientation(pq_ctx* ctx, int mode);
 
+/* Fused-chain element moves: -1 = automatic (a gather or scatter element
+ * pair with stride 1, every other offset even and an aligned tensor moves
+ * as one 16-byte access), 0 = always two 8-byte accesses.  Same results;
+ * read when a plan run launches its chains (eager runs, and the capture of
+ * a graph run). */
+int pq_set_chain_pairs(pq_ctx* ctx, int mode);
+
 /* ------------------------------------------------------------------------
  * Planned execution: a whole pair plan (contract_network!, src/layer2.jl:
  * 294-321) over every slice of a sliced network (src/layer2/slicing.jl),

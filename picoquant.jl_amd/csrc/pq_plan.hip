@@ -505,8 +505,8 @@ int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
         a.X = (const c64*)slot_ptr(c, p, st.a, group);
         a.Y = (c64*)slot_ptr(c, p, st.c, group);
         for (int t = 0; t < a.nst; ++t) a.st[t].S = (const c64*)slot_ptr(c, p, st.ops[t], group);
-        a.gvec = a.gvec_ok && ((uintptr_t)a.X % 16) == 0;
-        a.svec = a.svec_ok && ((uintptr_t)a.Y % 16) == 0;
+        a.gvec = c->chain_pairs != 0 && a.gvec_ok && ((uintptr_t)a.X % 16) == 0;
+        a.svec = c->chain_pairs != 0 && a.svec_ok && ((uintptr_t)a.Y % 16) == 0;
         ProfScope ps(c, PQ_KC_CHAIN, st.bytes, st.flops);
         hipError_t e = launch_chain(a, c->stream);
         if (e != hipSuccess) r = hipfail(c, e, "fused chain kernel");

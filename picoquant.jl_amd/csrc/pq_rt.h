@@ -78,6 +78,7 @@ struct pq_ctx {
   int override_kc = -1;
   int gemm_tile = -1;
   int gemm_orient = -1;
+  int chain_pairs = -1;   // fused chains: -1 automatic 16-byte pair moves, 0 never
   int arith = 0;
   void* scratch = nullptr;  // split-K partials for eager calls
   size_t scratch_bytes = 0;

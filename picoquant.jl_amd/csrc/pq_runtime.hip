@@ -836,6 +836,12 @@ int pq_set_gemm_orientation(pq_ctx* c, int mode) {
   return PQ_OK;
 }
 
+int pq_set_chain_pairs(pq_ctx* c, int mode) {
+  if (!c || mode < -1 || mode > 0) return PQ_ERR_ARG;
+  c->chain_pairs = mode;
+  return PQ_OK;
+}
+
 int pq_set_kernel_override(pq_ctx* c, int kc) {
   if (!c || kc < -1 || kc >= PQ_KC_COUNT) return PQ_ERR_ARG;
   c->override_kc = kc;

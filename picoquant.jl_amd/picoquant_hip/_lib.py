@@ -62,6 +62,7 @@ SIGNATURES = {
     "pq_set_kernel_override": (ct.c_int, [_P, ct.c_int]),
     "pq_set_gemm_tile": (ct.c_int, [_P, ct.c_int]),
     "pq_set_gemm_orientation": (ct.c_int, [_P, ct.c_int]),
+    "pq_set_chain_pairs": (ct.c_int, [_P, ct.c_int]),
     "pq_set_arithmetic": (ct.c_int, [_P, ct.c_int]),
     "pq_plan_create": (ct.c_int, [_P, _P, ct.POINTER(_P)]),
     "pq_plan_execute": (ct.c_int, [_P, _P, ct.c_int]),
@@ -230,6 +231,10 @@ class Context:
     def set_gemm_orientation(self, mode):
         """-1 automatic, 0 as given, 1 transposed (C^T = B^T A^T)."""
         self.check(self.lib.pq_set_gemm_orientation(self.ptr, int(mode)))
+
+    def set_chain_pairs(self, mode):
+        """Fused chains: -1 automatic 16-byte element-pair moves, 0 always two 8-byte moves."""
+        self.check(self.lib.pq_set_chain_pairs(self.ptr, int(mode)))
 
     def time_step(self, M, N, K, batch=1, kclass=-1, tile=-1, reps=3, dtype=None):
         """Median microseconds of one contraction shape (modes as (ext, s0, s1))."""

@@ -63,6 +63,29 @@ def test_fused_plan_matches_unfused_and_oracle(tile_bits):
     fused.close()
 
 
+def test_pair_moves_do_not_change_results():
+    """Gather / scatter element pairs as one 16-byte access or as two 8-byte
+    accesses (pq_set_chain_pairs 0): bit-identical amplitudes, at tile sizes
+    where chains take both forms (a scatter pair of stride 4 at 2^6)."""
+    tn = _net()
+    ctx = tn.backend.ctx
+    plan = PL.reconfigure(tn, PL.sweep_plan(tn, PL.grid_site_order(4, 4, "col")), windows=(8,))
+    for tile_bits in (6, 12):
+        limits = {"tile_bits": tile_bits, "min_elems": 64, "op_elems": 256}
+        fused = compile_plan(tn, plan, precision="mixed", small_elems=16, fuse=limits)
+        out = []
+        for mode in (-1, 0, -1):
+            ctx.set_chain_pairs(mode)
+            try:
+                fused.execute(False)   # eager: a captured graph keeps its first launch arguments
+                out.append(fused.result().copy())
+            finally:
+                ctx.set_chain_pairs(-1)
+        fused.close()
+        np.testing.assert_array_equal(out[0], out[1])
+        np.testing.assert_array_equal(out[0], out[2])
+
+
 def test_bad_chain_programs_are_rejected():
     from picoquant_hip import executor as E
     tn = _net()
