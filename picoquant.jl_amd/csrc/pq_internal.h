@@ -53,6 +53,7 @@ struct MultiDesc {
   void* C;
   Group M, N, K;      // as GenArgs
   Batch L;            // batch entries of one slice group (grid z)
+  int32_t conv;       // 1: element-type conversion of M.total elements A -> C (the other type)
   int32_t nhigh;
   int64_t hA[PQ_MAX_HIGH_BITS], hB[PQ_MAX_HIGH_BITS], hC[PQ_MAX_HIGH_BITS];
 };

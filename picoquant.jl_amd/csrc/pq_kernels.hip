@@ -12,6 +12,8 @@
 //              (materialises permuted / sliced views).
 #include "pq_internal.h"
 
+#include <type_traits>
+
 namespace {
 
 template <typename C>
@@ -197,7 +199,8 @@ __global__ __launch_bounds__(256) void k_copy(const C* __restrict__ In, C* __res
 
 // ----------------------------------------------------------------- k_multi
 // Grouped small contractions: problem blockIdx.y, batch entry blockIdx.z,
-// one lane per output element (grid-stride over blockIdx.x), loop over K.
+// one lane per output element (grid-stride over blockIdx.x), loop over K;
+// a problem may instead be a small element-type conversion.
 template <typename C>
 __global__ __launch_bounds__(256) void k_multi(const MultiDesc* __restrict__ descs, int64_t group) {
   const MultiDesc& d = descs[blockIdx.y];
@@ -210,6 +213,20 @@ __global__ __launch_bounds__(256) void k_multi(const MultiDesc* __restrict__ des
       ob += d.hB[j];
       oc += d.hC[j];
     }
+  if (d.conv) {
+    // element-type conversion (complex128 <-> complex64), same layout
+    typedef typename std::conditional<sizeof(C) == 16, c64, c128>::type D;
+    const C* A = reinterpret_cast<const C*>(d.A) + oa;
+    D* Out = reinterpret_cast<D*>(d.C) + oc;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < d.M.total; t += gridDim.x * blockDim.x) {
+      const C v = A[t];
+      D w;
+      w.x = (decltype(w.x))v.x;
+      w.y = (decltype(w.y))v.y;
+      Out[t] = w;
+    }
+    return;
+  }
   const C* A = reinterpret_cast<const C*>(d.A) + oa;
   const C* B = reinterpret_cast<const C*>(d.B) + ob;
   C* Out = reinterpret_cast<C*>(d.C) + oc;

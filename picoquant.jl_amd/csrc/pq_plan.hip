@@ -783,9 +783,19 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     return bad("sliced plan whose output does not depend on every sliced bond");
   // Launch sequences: by dependency level; the small contraction steps of a
   // level and element type share one grouped launch, the others launch alone.
+  // A small complex128 -> complex64 conversion joins the complex128 group
+  // of its level (the grouped kernel converts it element by element).
+  auto conv_elems = [&](const PStep& st) -> int64_t {
+    return p->slots[st.c].elems * p->slots[st.c].copies();
+  };
+  auto small_conv = [&](const PStep& st) {
+    return st.kind == 1 && p->slots[st.a].dtype == PQ_C128 && p->slots[st.c].dtype == PQ_C64 &&
+           conv_elems(st) <= 65536;
+  };
   auto small = [&](const PStep& st) {
-    return st.kind == 0 && (st.spec.kc == PQ_KC_STREAM || st.spec.kc == PQ_KC_GENERIC) &&
-           st.spec.Mt * st.spec.Nt <= 65536 && st.spec.Kt <= 64;
+    return (st.kind == 0 && (st.spec.kc == PQ_KC_STREAM || st.spec.kc == PQ_KC_GENERIC) &&
+            st.spec.Mt * st.spec.Nt <= 65536 && st.spec.Kt <= 64) ||
+           small_conv(st);
   };
   auto build_seq = [&](const std::vector<int>& order, std::vector<Launch>& seq) {
     std::vector<int> o = order;
@@ -797,7 +807,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
       size_t j = i;
       for (; j < o.size() && p->steps[o[j]].level == lvl; ++j) {
         const PStep& st = p->steps[o[j]];
-        const int ti = st.spec.dtype == PQ_C128 ? 0 : 1;
+        const int ti = (small_conv(st) || st.spec.dtype == PQ_C128) ? 0 : 1;
         if (small(st) && (int)grp[ti].size() < 65535) {
           grp[ti].push_back(o[j]);
         } else {
@@ -914,21 +924,34 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
         m.A = slot_ptr(c, p, st.a, 0);
         m.B = slot_ptr(c, p, st.b, 0);
         m.C = slot_ptr(c, p, st.c, 0);
-        if (!fill_group(m.M, st.spec.M) || !fill_group(m.N, st.spec.N) || !fill_group(m.K, st.spec.K)) {
-          free_plan(p);
-          return fail(c, PQ_ERR_UNSUPPORTED, "too many modes in a grouped step");
+        if (st.kind == 1) {
+          // conversion: conv_elems consecutive elements (every copy), one entry
+          const int64_t n = conv_elems(st);
+          m.conv = 1;
+          if (!fill_group(m.M, {{n, 1, 1}}) || !fill_group(m.N, {}) || !fill_group(m.K, {})) {
+            free_plan(p);
+            return fail(c, PQ_ERR_UNSUPPORTED, "grouped conversion too large");
+          }
+          m.L = single_batch();
+          maxw = std::max<uint64_t>(maxw, (uint64_t)n);
+        } else {
+          if (!fill_group(m.M, st.spec.M) || !fill_group(m.N, st.spec.N) || !fill_group(m.K, st.spec.K)) {
+            free_plan(p);
+            return fail(c, PQ_ERR_UNSUPPORTED, "too many modes in a grouped step");
+          }
+          m.L = st.spec.L;
+          maxw = std::max<uint64_t>(maxw, st.spec.Mt * st.spec.Nt);
+          maxb = std::max<unsigned>(maxb, st.spec.L.total);
         }
-        m.L = st.spec.L;
         m.nhigh = st.phase2 ? d->n_sliced - d->group_bits : 0;
         for (int j = 0; j < m.nhigh; ++j) {
           const int bit = d->group_bits + j;
           m.hA[j] = p->slots[st.a].slice_stride[bit];
           m.hB[j] = p->slots[st.b].slice_stride[bit];
-          m.hC[j] = 0;
+          // a conversion's output keeps the same sliced bonds as its input
+          m.hC[j] = st.kind == 1 ? p->slots[st.c].slice_stride[bit] : 0;
         }
         hd.push_back(m);
-        maxw = std::max<uint64_t>(maxw, st.spec.Mt * st.spec.Nt);
-        maxb = std::max<unsigned>(maxb, st.spec.L.total);
       }
       L.blocks_x = (unsigned)std::min<uint64_t>((maxw + 255) / 256, 64);
       L.batch = maxb;
