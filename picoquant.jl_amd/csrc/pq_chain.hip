@@ -238,37 +238,43 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
       buf[q ^ a.gpa] = lc{P[2 * it + 1].x, P[2 * it + 1].y};
     }
   };
-  auto scatter = [&](const lc* buf, uint32_t t) {
-    int32_t ox, oy;
-    group_offsets(a.spec, t, ox, oy);
-    c64* dst = a.Y + oy + sg0;
-    lc v[2 * CH_NIT];
-    int o[CH_NIT];
+  // scatter in two halves: the finished tile is read from LDS into
+  // registers (readq), and its stores are issued later (storeq), while the
+  // next tile's first stage runs
+  lc Q[2 * CH_NIT];
+  auto readq = [&](const lc* buf) {
 #pragma unroll
     for (int it = 0; it < CH_NIT; ++it) {
       const int q = sp0 ^ hid[2 * CH_NIT + it];
-      v[2 * it] = buf[q];
-      v[2 * it + 1] = buf[q ^ a.spa];
-      o[it] = hid[3 * CH_NIT + it];
+      Q[2 * it] = buf[q];
+      Q[2 * it + 1] = buf[q ^ a.spa];
     }
+  };
+  auto storeq = [&](uint32_t t) {
+    int32_t ox, oy;
+    group_offsets(a.spec, t, ox, oy);
+    c64* dst = a.Y + oy + sg0;
+    int o[CH_NIT];
+#pragma unroll
+    for (int it = 0; it < CH_NIT; ++it) o[it] = hid[3 * CH_NIT + it];
     if (a.svec) {
 #pragma unroll
       for (int it = 0; it < CH_NIT; ++it) {
         float4 w;
-        w.x = v[2 * it].x;
-        w.y = v[2 * it].y;
-        w.z = v[2 * it + 1].x;
-        w.w = v[2 * it + 1].y;
+        w.x = Q[2 * it].x;
+        w.y = Q[2 * it].y;
+        w.z = Q[2 * it + 1].x;
+        w.w = Q[2 * it + 1].y;
         *reinterpret_cast<float4*>(dst + o[it]) = w;
       }
     } else {
 #pragma unroll
       for (int it = 0; it < CH_NIT; ++it) {
         c64 w0, w1;
-        w0.x = v[2 * it].x;
-        w0.y = v[2 * it].y;
-        w1.x = v[2 * it + 1].x;
-        w1.y = v[2 * it + 1].y;
+        w0.x = Q[2 * it].x;
+        w0.y = Q[2 * it].y;
+        w1.x = Q[2 * it + 1].x;
+        w1.y = Q[2 * it + 1].y;
         dst[o[it]] = w0;
         dst[o[it] + a.sps] = w1;
       }
@@ -283,11 +289,15 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
   __syncthreads();
 
   // every wave of a workgroup runs the same iterations, and the loop ends
-  // when the workgroup's tiles run out.  The next tile's loads are issued
-  // after the first stage (the memory pipeline is idle during the stages,
-  // and the previous tile's stores have drained by then), its data is put
-  // into the tile after this tile's scatter.
+  // when the workgroup's tiles run out.  Per tile: the previous tile's
+  // stores are issued, then the stages run (the next tile's loads are
+  // issued after the first: the memory pipeline is otherwise idle during
+  // the stages); the finished tile is read into registers and the
+  // prefetched one put into LDS.
+  bool held = false;    // Q holds tile tq, not yet stored
+  uint32_t tq = 0;
   for (; t < total; t += G) {
+    if (held) storeq(tq);
     for (int s = 0; s < a.nst; ++s) {
       const ChainStage& g = a.st[s];
       const int nkc = ((1 << g.nk) + 3) / 4, nmb = ((1 << g.nm) + 15) / 16;
@@ -304,11 +314,14 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
       if (s == 0 && t + G < total) load(t + G);
       __syncthreads();
     }
-    scatter(tile, t);
+    readq(tile);
+    held = true;
+    tq = t;
     __syncthreads();
     if (t + G < total) put(tile);
     __syncthreads();
   }
+  if (held) storeq(tq);
 }
 
 }  // namespace
