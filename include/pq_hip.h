@@ -86,6 +86,19 @@ int pq_view(pq_ctx* ctx, pq_tensor t, int axis, int64_t lo, int64_t hi, pq_tenso
  * and by callers that need dense storage). */
 int pq_materialize(pq_ctx* ctx, pq_tensor t, pq_tensor* out);
 
+/* decompose_tensor! -> decompose_tensor (src/backends/interactive.jl:130-152,
+ * src/layer1.jl:146-190): SVD split of t into B (dims of the 1-based `left`
+ * axes, then chi) and C (chi, then the `right` axes) with B C = t permuted to
+ * (left, right) up to the discarded singular values; S is split as
+ * sqrt(S) into both.  chi counts the singular values whose ratio to the
+ * spectrum's 2-norm exceeds max(threshold, sqrt(eps)) of the element type,
+ * capped by max_rank when max_rank > 0.  t is not consumed (the caller
+ * frees it, as the backend verb does).  On the GPU: rocSOLVER gesvd;
+ * synchronises the context's stream (chi is needed on the host).  A tensor
+ * with no singular value above the threshold gives PQ_ERR_UNSUPPORTED. */
+int pq_decompose(pq_ctx* ctx, pq_tensor t, int nleft, const int32_t* left, int nright, const int32_t* right,
+                 double threshold, int max_rank, pq_tensor* B, pq_tensor* C, int* chi);
+
 /* Device pointer and element offset of a tensor (for in-process callers that
  * share device memory, e.g. tests comparing against torch). */
 int pq_tensor_device_ptr(pq_ctx* ctx, pq_tensor t, void** base, int64_t* elem_offset,

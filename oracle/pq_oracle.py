@@ -20,6 +20,13 @@ Pins (what checks each part; see tests/test_oracle.py):
   * bgreedy tau->0 ............................ plan validity + ledger re-count
   * optimal_contraction_cost .................. hand-counted costs of tiny chains / stars
   * gate tensors .............................. closed forms + unitarity
+  * decompose_tensor / MPS drivers ............ exact (untruncated) MPS == brute-force
+                                                state vector; GHZ-5 closed form and
+                                                MPSState amplitudes (test/layer2_tests.jl:
+                                                300-347); Eckart-Young residual of a
+                                                truncated split; chain compression to one
+                                                bond of dimension 2 (:241-295); threshold /
+                                                max_rank bond dimensions (:349-416)
 """
 
 from __future__ import annotations
@@ -754,3 +761,179 @@ def qft_closed_form(n, x):
 
 def bits_to_config(bits):
     return "".join("1" if b else "0" for b in bits)
+
+
+# -- MPS and compression (src/layer2.jl:423-703, src/mps.jl) -------------------
+
+def inneighbours(tn, node_label):
+    """src/layer3.jl:324-336: nodes feeding node_label through non-virtual edges."""
+    out = []
+    for ix in tn.nodes[node_label].indices:
+        e = tn.edges[ix]
+        if e.src is not None and e.src != node_label and not e.virtual:
+            out.append(e.src)
+    return out
+
+
+def decompose_tensor_network(tn, node_label, left_indices, right_indices, threshold=1e-13, max_rank=0,
+                             left_label=None, right_label=None):
+    """decompose_tensor! (src/layer2.jl:482-560): split a node into B (left
+    indices + new virtual bond) and C (bond + right indices) and rewire."""
+    node = tn.nodes[node_label]
+    pos = {v: k + 1 for k, v in enumerate(node.indices)}
+    lp = [pos[x] for x in left_indices]
+    rp = [pos[x] for x in right_indices]
+    b_label = new_label(tn, "node") if left_label is None else left_label
+    c_label = new_label(tn, "node") if right_label is None else right_label
+    chi = tn.backend.decompose_tensor(node_label, lp, rp, threshold=threshold, max_rank=max_rank,
+                                      left_label=b_label, right_label=c_label)
+    b_dims = [node.dims[i - 1] for i in lp]
+    c_dims = [node.dims[i - 1] for i in rp]
+    if chi > 0:
+        vdim = chi
+    else:
+        vdim = min(math.prod(b_dims), math.prod(c_dims))
+        if max_rank > 0:
+            vdim = min(vdim, max_rank)
+    index_label = new_label(tn, "index")
+    tn.nodes[b_label] = Node(list(left_indices) + [index_label], b_dims + [vdim], b_label)
+    tn.nodes[c_label] = Node([index_label] + list(right_indices), [vdim] + c_dims, c_label)
+    for ix in left_indices:
+        e = tn.edges[ix]
+        if e.src == node_label:
+            e.src = b_label
+        elif e.dst == node_label:
+            e.dst = b_label
+    for ix in right_indices:
+        e = tn.edges[ix]
+        if e.src == node_label:
+            e.src = c_label
+        elif e.dst == node_label:
+            e.dst = c_label
+    tn.edges[index_label] = Edge(b_label, c_label, None, True)
+    # the backend verb has already deleted the split tensor's data
+    if node_label not in (b_label, c_label):
+        del tn.nodes[node_label]
+    return b_label, c_label
+
+
+def compress_bond(tn, node_1, node_2, threshold=1e-13, max_rank=0):
+    """compress_bond! (src/layer2.jl:451-468): contract the pair, split it again."""
+    left = tn.nodes[node_1]
+    right = tn.nodes[node_2]
+    left_indices = [x for x in left.indices if x not in right.indices]
+    right_indices = [x for x in right.indices if x not in left.indices]
+    combined = contract_pair(tn, node_1, node_2)
+    return decompose_tensor_network(tn, combined, left_indices, right_indices, threshold=threshold,
+                                    max_rank=max_rank, left_label=node_1, right_label=node_2)
+
+
+def compress_tensor_chain(tn, nodes, threshold=1e-13, max_rank=0):
+    """compress_tensor_chain! (src/layer2.jl:423-441): forward then backward pass."""
+    for i in range(len(nodes) - 1):
+        compress_bond(tn, nodes[i], nodes[i + 1], threshold, max_rank)
+    for i in range(len(nodes) - 2, -1, -1):
+        compress_bond(tn, nodes[i], nodes[i + 1], threshold, max_rank)
+
+
+def _gate_layers(tn):
+    layer_nodes = {}
+    for k, v in tn.node_layers.items():
+        layer_nodes.setdefault(v, []).append(k)
+    layers = sorted(k for k in layer_nodes if k > 0)
+    if -1 in layer_nodes:
+        layers.append(-1)
+    return layers, layer_nodes
+
+
+def contract_mps_tensor_network_circuit(tn, threshold=1e-13, max_rank=0, pbc=False):
+    """contract_mps_tensor_network_circuit! (src/layer2.jl:570-625): absorb each
+    gate layer into the MPS nodes, then compress the bonds between the
+    updated neighbours."""
+    mps_nodes = [tn.edges[x].src for x in tn.input_qubits]
+    if any(x is None for x in mps_nodes):
+        raise ValueError("Input qubit values must be set")
+    layers, layer_nodes = _gate_layers(tn)
+    for layer in layers:
+        updated = []
+        for node in layer_nodes[layer]:
+            input_node = inneighbours(tn, node)[0]
+            idx = mps_nodes.index(input_node)
+            mps_nodes[idx] = contract_pair(tn, input_node, node)
+            updated.append(idx)
+        updated.sort()
+        for i in range(len(updated) - 1):
+            dist = updated[i + 1] - updated[i]
+            if pbc:
+                dist = min(dist, updated[i] + len(mps_nodes) - updated[i + 1])
+            if dist != 1:
+                raise ValueError("Gates between non-neighboring qubits")
+            compress_bond(tn, mps_nodes[updated[i]], mps_nodes[updated[i + 1]], threshold, max_rank)
+    for node in mps_nodes:
+        tn.backend.save_output(node, node)
+    return mps_nodes
+
+
+def calculate_mps_amplitudes(tn, mps_nodes, result="result"):
+    """calculate_mps_amplitudes! (src/layer2.jl:634-645)."""
+    out = mps_nodes[0]
+    for node in mps_nodes[1:]:
+        out = contract_pair(tn, out, node)
+    tn.backend.permute_tensor(out, list(tn.qubit_ordering))
+    tn.backend.reshape_tensor(out, [list(range(1, len(mps_nodes) + 1))])
+    tn.backend.save_output(out, result)
+    return out
+
+
+def contract_tensor_network_circuit_with_compression(tn, threshold=1e-13, max_rank=0):
+    """contract_tensor_network_circuit_with_compression! (src/layer2.jl:655-703):
+    as the MPS driver, compressing consecutive updated nodes without the
+    neighbour check."""
+    nodes = [tn.edges[x].src for x in tn.input_qubits]
+    if any(x is None for x in nodes):
+        raise ValueError("Input qubit values must be set")
+    layers, layer_nodes = _gate_layers(tn)
+    for layer in layers:
+        updated = []
+        for node in layer_nodes[layer]:
+            input_node = inneighbours(tn, node)[0]
+            idx = nodes.index(input_node)
+            nodes[idx] = contract_pair(tn, input_node, node)
+            updated.append(idx)
+        for i in range(len(updated) - 1):
+            compress_bond(tn, nodes[updated[i]], nodes[updated[i + 1]], threshold, max_rank)
+    for node in nodes:
+        tn.backend.save_output(node, node)
+    return nodes
+
+
+def mps_amplitude(tn, mps_nodes, conf):
+    """MPSState(...)[conf] (src/mps.jl:25-55, 87-110): select each node's output
+    index at a configuration bit and contract the chain left to right.  The
+    bit read for node k is the one the reference's ordering names:
+    ordering[qubit_ordering[k]] = (position of node k's output index within
+    node k) (src/mps.jl:51-54), kept as written for parity; it is node k's
+    own bit only where that position equals k."""
+    n = len(mps_nodes)
+    sel_nodes, sel_data, out_pos = [], [], []
+    for label in mps_nodes:
+        node = tn.nodes[label]
+        p = next(k for k, x in enumerate(node.indices) if x in tn.output_qubits)
+        other = [k for k in range(len(node.indices)) if k != p]
+        t = np.transpose(np.asarray(tn.backend.load_tensor_data(label)), [p] + other)
+        sel_nodes.append(Node([node.indices[k] for k in other], [node.dims[k] for k in other], label))
+        sel_data.append((t[0], t[1]))
+        out_pos.append(p)
+    ordering = [0] * n
+    for i, e in zip(out_pos, tn.qubit_ordering):
+        ordering[e - 1] = i + 1          # 1-based position, as in the reference
+    bits = [0 if ch == "0" else 1 for ch in conf]
+    n1 = sel_nodes[0]
+    d1 = sel_data[0][bits[ordering[0] - 1]]
+    for idx in range(1, n):
+        n2 = sel_nodes[idx]
+        common, remaining = sort_indices(n1, n2)
+        ia, ib = create_ncon_indices(n1, n2, common, remaining)
+        d1 = contract_tensors_data(d1, ia, sel_data[idx][bits[ordering[idx] - 1]], ib)
+        n1 = Node(remaining, [1] * len(remaining), "n1")
+    return complex(np.asarray(d1).reshape(-1)[0])

@@ -43,7 +43,10 @@ def build(verbose: bool = False) -> pathlib.Path:
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = OUT.with_suffix(".so.tmp")
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp)]
+    # rocSOLVER (the SVD of decompose_tensor) and rocBLAS, found through the rpath
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp),
+           f"-L{rocm_lib}", f"-Wl,-rpath,{rocm_lib}", "-lrocsolver", "-lrocblas"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

@@ -82,6 +82,7 @@ struct pq_ctx {
   int arith = 0;
   void* scratch = nullptr;  // split-K partials for eager calls
   size_t scratch_bytes = 0;
+  void* blas = nullptr;     // rocBLAS handle on this stream (rocSOLVER SVD), created on first use
 };
 
 namespace pqrt {
@@ -95,6 +96,8 @@ void unref(pq_ctx* c, Buf* b);
 Tensor* get(pq_ctx* c, pq_tensor h);
 pq_tensor add(pq_ctx* c, const Tensor& t);
 int materialize_into(pq_ctx* c, const Tensor& src, Tensor& dst);
+// Destroys the context's rocBLAS handle, if one was created.
+void release_blas(pq_ctx* c);
 
 // Choose kernel class and tile for a step (honouring the context overrides
 // unless the step already names them).  Returns PQ_OK or PQ_ERR_UNSUPPORTED.

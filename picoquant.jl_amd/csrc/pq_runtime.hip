@@ -495,6 +495,7 @@ int pq_ctx_destroy(pq_ctx* c) {
   for (auto& kv : c->free_blocks) hipFree(kv.second);
   c->free_blocks.clear();
   if (c->scratch) hipFree(c->scratch);
+  release_blas(c);
   for (auto& p : c->pe)
     for (auto& ev : p.events) {
       hipEventDestroy(ev.first);

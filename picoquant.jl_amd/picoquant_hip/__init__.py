@@ -17,6 +17,9 @@ from .optimal_order import netcon, netcon_contraction
 from .network import network_from_json, to_json
 from .qasm import convert_qiskit_circ_to_network, load_qasm_as_circuit, load_qasm_as_circuit_from_file
 from .dsl import DSLBackend, execute_dsl_file
+from .mps import (MPSState, calculate_mps_amplitudes, compress_bond, compress_tensor_chain,
+                  contract_mps_tensor_network_circuit, contract_tensor_network_circuit_with_compression,
+                  decompose_tensor)
 from . import circuits
 
 __all__ = [
@@ -28,4 +31,7 @@ __all__ = [
     "partition_network_on_virtual_bonds", "replace_with_view", "slice_tensor_network", "circuits",
     "netcon", "netcon_contraction", "network_from_json", "to_json", "convert_qiskit_circ_to_network",
     "load_qasm_as_circuit", "load_qasm_as_circuit_from_file", "DSLBackend", "execute_dsl_file",
+    "MPSState", "calculate_mps_amplitudes", "compress_bond", "compress_tensor_chain",
+    "contract_mps_tensor_network_circuit", "contract_tensor_network_circuit_with_compression",
+    "decompose_tensor",
 ]

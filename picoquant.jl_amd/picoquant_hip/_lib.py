@@ -52,6 +52,8 @@ SIGNATURES = {
     "pq_tensor_free": (ct.c_int, [_P, ct.c_int64]),
     "pq_contract": (ct.c_int, [_P, ct.c_int64, ct.c_int, _I32P, ct.c_int64, ct.c_int, _I32P, _I64P]),
     "pq_permute": (ct.c_int, [_P, ct.c_int64, _I32P, _I64P]),
+    "pq_decompose": (ct.c_int, [_P, ct.c_int64, ct.c_int, _I32P, ct.c_int, _I32P, ct.c_double, ct.c_int,
+                                _I64P, _I64P, ct.POINTER(ct.c_int)]),
     "pq_reshape": (ct.c_int, [_P, ct.c_int64, ct.c_int, _I64P]),
     "pq_view": (ct.c_int, [_P, ct.c_int64, ct.c_int, ct.c_int64, ct.c_int64, _I64P]),
     "pq_materialize": (ct.c_int, [_P, ct.c_int64, _I64P]),
@@ -196,6 +198,14 @@ class Context:
         self.check(self.lib.pq_contract(self.ptr, ha, len(ia), i32arr(ia), hb, len(ib), i32arr(ib),
                                         ct.byref(out)))
         return out.value
+
+    def decompose(self, h, left, right, threshold=1e-13, max_rank=0):
+        """SVD split (pq_decompose): (B, C, chi) for 1-based axis lists."""
+        b, c, chi = ct.c_int64(), ct.c_int64(), ct.c_int()
+        self.check(self.lib.pq_decompose(self.ptr, h, len(left), i32arr(left or [0]), len(right),
+                                         i32arr(right or [0]), float(threshold), int(max_rank),
+                                         ct.byref(b), ct.byref(c), ct.byref(chi)))
+        return b.value, c.value, chi.value
 
     def permute(self, h, perm):
         out = ct.c_int64()

@@ -121,35 +121,16 @@ class HIPBackend(AbstractBackend):
 
     def decompose_tensor(self, tensor, left_positions, right_positions, threshold=1e-13,
                          max_rank=0, left_label=None, right_label=None):
-        """src/backends/interactive.jl:130-152 -> src/layer1.jl:384-422.
+        """src/backends/interactive.jl:130-152 -> src/layer1.jl:146-190.
 
-        The SVD split runs on the host (LAPACK through numpy): it only serves
-        compression, off the contraction path.
+        On the GPU (pq_decompose: rocSOLVER gesvd, B = U sqrt(S), C = sqrt(S) V^H);
+        returns chi, the new bond's dimension.
         """
-        t = self.load_tensor_data(tensor)
-        B, C, chi = svd_split(t, list(left_positions), list(right_positions), threshold, max_rank)
-        self.save_tensor_data(left_label, B)
-        self.save_tensor_data(right_label, C)
-        self.delete_tensor(tensor)
+        h = self.tensors[str(tensor)]
+        hb, hc, chi = self.ctx.decompose(h, list(left_positions), list(right_positions), threshold, max_rank)
+        self._set(str(left_label), hb)
+        self._set(str(right_label), hc)
+        if str(tensor) not in (str(left_label), str(right_label)):
+            self.delete_tensor(tensor)
         return chi
 
-
-def svd_split(t, left, right, threshold=1e-13, max_rank=0):
-    """src/layer1.jl:384-422: permute to (left, right), SVD, keep singular values
-    whose ratio to the spectrum norm exceeds max(threshold, sqrt(eps))."""
-    dims = t.shape
-    ld = [dims[x - 1] for x in left]
-    rd = [dims[x - 1] for x in right]
-    A = np.reshape(np.transpose(t, [x - 1 for x in left + right]),
-                   (math.prod(ld), math.prod(rd)), order="F")
-    U, S, Vh = np.linalg.svd(A, full_matrices=False)
-    eps = np.finfo(np.float64 if t.dtype == np.complex128 else np.float32).eps
-    thr = max(threshold, math.sqrt(eps))
-    norm = math.sqrt(float(np.sum(S ** 2)))
-    chi = int(np.sum(S / norm > thr))
-    if max_rank > 0:
-        chi = min(max_rank, chi)
-    s = np.sqrt(S[:chi])
-    B = np.reshape(U[:, :chi] * s[None, :], tuple(ld + [chi]), order="F")
-    C = np.reshape(s[:, None] * Vh[:chi, :], tuple([chi] + rd), order="F")
-    return B, C, chi

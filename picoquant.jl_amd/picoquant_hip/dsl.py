@@ -27,7 +27,7 @@ import pathlib
 import numpy as np
 
 from . import _lib
-from .backend import AbstractBackend, Metrics, svd_split
+from .backend import AbstractBackend, Metrics
 
 
 class TensorStore:
@@ -183,11 +183,11 @@ def execute_dsl_file(dtype=np.complex64, dsl_filename="contract_network.tl", ten
                 elif op == "decompose":
                     a, bl, bi, cl, ci = cmd[1:6]
                     opts = json.loads(" ".join(cmd[6:]) or "{}")
-                    B, C, _ = svd_split(ctx.download(tensors[a]), [int(x) for x in bi.split(",")],
-                                        [int(x) for x in ci.split(",")], float(opts.get("threshold", 1e-13)),
-                                        int(opts.get("max_rank", 0)))
-                    put(bl, ctx.upload(B))
-                    put(cl, ctx.upload(C))
+                    hb, hc, _ = ctx.decompose(tensors[a], [int(x) for x in bi.split(",")],
+                                              [int(x) for x in ci.split(",")], float(opts.get("threshold", 1e-13)),
+                                              int(opts.get("max_rank", 0)))
+                    put(bl, hb)
+                    put(cl, hc)
                 else:
                     raise ValueError(f"line {lineno}: unknown DSL command {op!r}")
         return sorted(tensors)
