@@ -128,17 +128,20 @@ function permute_tensor(b::HIPBackend, tensor::Symbol, axes::Array{Int, 1})
     pq_set!(b, tensor, out[])
 end
 
-"decompose_tensor! (src/backends/interactive.jl:130-152): SVD split on the host."
+"decompose_tensor! (src/backends/interactive.jl:130-152): SVD split on the GPU (pq_decompose)."
 function decompose_tensor!(b::HIPBackend{T}, tensor::Symbol, left_positions::Array{Int, 1},
                            right_positions::Array{Int, 1}; threshold::AbstractFloat=1e-13,
                            max_rank::Int=0, left_label::Symbol, right_label::Symbol) where {T}
-    data = load_tensor_data(b, tensor)
-    (B, C, chi) = decompose_tensor(data, left_positions, right_positions;
-                                   threshold=threshold, max_rank=max_rank)
-    save_tensor_data(b, left_label, B)
-    save_tensor_data(b, right_label, C)
-    delete_tensor!(b, tensor)
-    chi
+    hb, hc, chi = Ref{Int64}(0), Ref{Int64}(0), Ref{Cint}(0)
+    pq_check(b, ccall((:pq_decompose, LIBPQ), Cint,
+                      (Ptr{Cvoid}, Int64, Cint, Ptr{Int32}, Cint, Ptr{Int32}, Cdouble, Cint,
+                       Ref{Int64}, Ref{Int64}, Ref{Cint}),
+                      b.ctx, b.tensors[tensor], length(left_positions), Int32.(left_positions),
+                      length(right_positions), Int32.(right_positions), threshold, max_rank, hb, hc, chi))
+    pq_set!(b, left_label, hb[])
+    pq_set!(b, right_label, hc[])
+    tensor in (left_label, right_label) || delete_tensor!(b, tensor)
+    Int(chi[])
 end
 
 "delete_tensor! (src/backends/interactive.jl:159-161)."
