@@ -141,6 +141,46 @@ def grid7x7_config():
     return out
 
 
+def mps_chain_config():
+    """MPS contraction (contract_mps_tensor_network_circuit!, src/layer2.jl:570-625,
+    the reference's own GPU benchmark target): a 1 x 20 chain random circuit
+    (the grid construction on one row, so every coupler joins neighbours),
+    depth 64, seed 2026, bond dimension capped at 256, ComplexF64.  GPU: the
+    drivers with pq_contract / pq_decompose (rocSOLVER), second of two runs;
+    CPU: the numpy oracle's restatement, one run.  Both time the driver only
+    (network construction excluded).  The GPU amplitudes are checked against
+    the oracle's run with the same truncation."""
+    import picoquant_hip as P
+    from picoquant_hip import circuits as C
+    from oracle import pq_oracle as O
+    n, depth, seed, chi = 20, 64, 2026, 256
+    t_gpu, psi, bond = None, None, 0
+    for _ in range(2):
+        tn = P.network_from_gates(n, C.grid_circuit_gates(1, n, depth, seed), backend=P.HIPBackend(np.complex128),
+                                  decompose=True)
+        P.add_input(tn, "0" * n)
+        ctx = tn.backend.ctx
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        nodes = P.contract_mps_tensor_network_circuit(tn, max_rank=chi)
+        ctx.synchronize()
+        t_gpu = time.perf_counter() - t0
+        bond = max(d for v in nodes for x, d in zip(tn.nodes[v].indices, tn.nodes[v].dims) if tn.edges[x].virtual)
+        P.calculate_mps_amplitudes(tn, nodes)
+        psi = np.asarray(tn.backend.load_tensor_data("result")).reshape(-1, order="F")
+    to = O.network_from_gates(n, O.grid_circuit_gates(1, n, depth, seed), decompose=True)
+    O.add_input(to, "0" * n)
+    t0 = time.perf_counter()
+    onodes = O.contract_mps_tensor_network_circuit(to, max_rank=chi)
+    t_cpu = time.perf_counter() - t0
+    O.calculate_mps_amplitudes(to, onodes)
+    ref = np.asarray(to.backend.load_tensor_data("result")).reshape(-1, order="F")
+    return {"workload": f"MPS of a 1x{n} chain random circuit, depth {depth} (seed {seed}), bond cap {chi}, "
+                        "complex128", "gpu_s": round(t_gpu, 4), "cpu_oracle_s": round(t_cpu, 3),
+            "speedup": round(t_cpu / t_gpu, 2), "max_bond": int(bond),
+            "rel_err_vs_oracle": float(np.linalg.norm(psi - ref) / np.linalg.norm(ref))}
+
+
 def choose_plan(tn, open_q, model=None):
     from picoquant_hip import learned
     return learned.select_plan(tn, ROWS, COLS, open_q, N_SLICED, model=model)
@@ -268,6 +308,7 @@ def main():
                     help="reuse the autotuner's per-step choices from this file (written when missing), "
                          "so profiler runs trace only the plan's own kernels")
     ap.add_argument("--no-grid7", action="store_true", help="skip the 7x7 depth-12 precision comparison")
+    ap.add_argument("--no-mps", action="store_true", help="skip the MPS chain contraction timing")
     ap.add_argument("--no-fp64-check", action="store_true",
                     help="skip the FP64 reference run (and the FP16-split variant that is checked against it); "
                          "for profiler runs that should hold only the complex64 plan's kernels")
@@ -413,6 +454,10 @@ def main():
     if not args.no_grid7:
         grid7 = grid7x7_config()
         log(f"[bench] 7x7 d12: {grid7}")
+    mps = None
+    if not args.no_mps:
+        mps = mps_chain_config()
+        log(f"[bench] MPS chain: {mps}")
     plain = None
     if not args.no_plain:
         t_slice, res_slice = plain_fp64_slice_time(plan, sliced, open_q)
@@ -452,6 +497,7 @@ def main():
         "planner": planner_info,
         "fp16_split_variant": split16,
         "grid7x7_d12": grid7,
+        "mps_chain20": mps,
     }
     print(json.dumps(out))
 

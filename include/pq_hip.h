@@ -93,7 +93,7 @@ int pq_materialize(pq_ctx* ctx, pq_tensor t, pq_tensor* out);
  * sqrt(S) into both.  chi counts the singular values whose ratio to the
  * spectrum's 2-norm exceeds max(threshold, sqrt(eps)) of the element type,
  * capped by max_rank when max_rank > 0.  t is not consumed (the caller
- * frees it, as the backend verb does).  On the GPU: rocSOLVER gesvd;
+ * frees it, as the backend verb does).  On the GPU: rocSOLVER gesdd;
  * synchronises the context's stream (chi is needed on the host).  A tensor
  * with no singular value above the threshold gives PQ_ERR_UNSUPPORTED. */
 int pq_decompose(pq_ctx* ctx, pq_tensor t, int nleft, const int32_t* left, int nright, const int32_t* right,

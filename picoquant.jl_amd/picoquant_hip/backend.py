@@ -123,7 +123,7 @@ class HIPBackend(AbstractBackend):
                          max_rank=0, left_label=None, right_label=None):
         """src/backends/interactive.jl:130-152 -> src/layer1.jl:146-190.
 
-        On the GPU (pq_decompose: rocSOLVER gesvd, B = U sqrt(S), C = sqrt(S) V^H);
+        On the GPU (pq_decompose: rocSOLVER gesdd, B = U sqrt(S), C = sqrt(S) V^H);
         returns chi, the new bond's dimension.
         """
         h = self.tensors[str(tensor)]
