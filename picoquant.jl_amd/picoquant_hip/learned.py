@@ -26,11 +26,14 @@ def _perturbed(order, rng, swaps, keep_last):
     return [q for q in o if q not in ks] + [q for q in o if q in ks]
 
 
-def candidate_plans(tn, rows, cols, open_q, n_candidates=64, seed=0, n_greedy=2, n_reconf=2):
+def candidate_plans(tn, rows, cols, open_q, n_candidates=64, seed=0, n_greedy=2, n_reconf=2, n_partition=0):
     """[(name, plan)]: n_candidates candidate pair plans for a grid network.
     The column and row sweeps also enter after subtree reconfiguration
     (planner.reconfigure, "-rc"), and so do the first n_reconf seeded
-    perturbations of them ("-rcp<k>", one window size)."""
+    perturbations of them ("-rcp<k>", one window size).  n_partition > 0
+    adds that many top-down partition plans (planner.partition_plan, one
+    seed each, "partition-<k>"); off by default: on grid circuits their cost
+    is far above the sweeps' (balanced cuts cross every qubit line)."""
     rng = np.random.default_rng(seed)
     base = {"col": PL.grid_site_order(rows, cols, "col", open_last=open_q),
             "row": PL.grid_site_order(rows, cols, "row", open_last=open_q)}
@@ -45,6 +48,8 @@ def candidate_plans(tn, rows, cols, open_q, n_candidates=64, seed=0, n_greedy=2,
     for k, v in base.items():
         rev = list(reversed([q for q in v if q not in set(open_q)])) + [q for q in v if q in set(open_q)]
         cands.append((f"sweep-{k}-rev", PL.sweep_plan(tn, rev)))
+    for k in range(n_partition):
+        cands.append((f"partition-{k}", PL.partition_plan(tn, imbalance=0.25, leaf=24, seed=seed + k)))
     plan0, _, _ = PL.bgreedy_single(tn, 1.0, 0.0)
     cands.append(("bgreedy-tau0", plan0))
     for g in range(n_greedy):

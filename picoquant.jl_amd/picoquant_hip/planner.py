@@ -360,6 +360,156 @@ def reconfigure(network, plan, windows=(12, 16), rounds=12, max_work=1 << 24):
     return out
 
 
+def _bisect(nodes, adj, imbalance, rng):
+    """Split `nodes` into two parts of balanced size (each within
+    (1/2 +- imbalance) of the total) with a small cut: the sign of the
+    Fiedler vector of the bond-weighted Laplacian (weight log2 of the bond
+    dimension) split at the median, then Kernighan-Lin passes moving single
+    boundary nodes while the cut shrinks and the balance holds."""
+    n = len(nodes)
+    pos = {v: i for i, v in enumerate(nodes)}
+    L = np.zeros((n, n))
+    for v in nodes:
+        i = pos[v]
+        for u, w in adj[v].items():
+            j = pos.get(u)
+            if j is not None:
+                L[i, j] -= w
+                L[i, i] += w
+    L += 1e-9 * np.eye(n)
+    _, vecs = np.linalg.eigh(L)
+    f = vecs[:, 1] + 1e-12 * rng.standard_normal(n)
+    order = np.argsort(f)
+    side = np.zeros(n, dtype=bool)
+    side[order[n // 2:]] = True
+    lo, hi = math.ceil((0.5 - imbalance) * n), math.floor((0.5 + imbalance) * n)
+
+    def gain(i):
+        # cut decrease when node i changes side
+        g = 0.0
+        for u, w in adj[nodes[i]].items():
+            j = pos.get(u)
+            if j is not None:
+                g += w if side[j] != side[i] else -w
+        return g
+    for _ in range(8):
+        moved = False
+        for i in sorted(range(n), key=lambda i: -gain(i)):
+            g = gain(i)
+            if g <= 1e-12:
+                continue
+            cnt = int(side.sum())
+            new = cnt - 1 if side[i] else cnt + 1
+            if lo <= new <= hi:
+                side[i] = not side[i]
+                moved = True
+        if not moved:
+            break
+    a = [nodes[i] for i in range(n) if not side[i]]
+    b = [nodes[i] for i in range(n) if side[i]]
+    return a, b
+
+
+def partition_plan(network, imbalance=0.1, leaf=20, max_work=1 << 22, seed=0):
+    """Top-down plan: the network is bisected (balanced parts with a small
+    bond-weighted cut, _bisect), each part planned recursively, and parts of
+    fewer than `leaf` tensors contracted in their optimal order (pq_netcon);
+    a part whose exact search exceeds max_work is ordered greedily.  The
+    idea of the reference's KaHyPar_contraction_plan
+    (src/layer2/KaHyPar_contraction.jl:31-93: k-way partition, netcon below
+    V nodes), with a spectral / Kernighan-Lin bisection in place of the
+    KaHyPar library (absent here; that file is not part of the reference's
+    package).  Returns a pair plan."""
+    from .optimal_order import optimal_tree
+    from ._lib import PQError
+
+    rng = np.random.default_rng(seed)
+    ids, ext, leaves = {}, {}, {}
+    for lab, node in network.nodes.items():
+        m = 0
+        for x, d in zip(node.indices, node.dims):
+            if x not in ids:
+                ids[x] = len(ids)
+                ext[ids[x]] = int(d)
+            m |= 1 << ids[x]
+        leaves[lab] = m
+    owners = {}
+    for lab, node in network.nodes.items():
+        for x in node.indices:
+            owners.setdefault(x, []).append(lab)
+    adj = {lab: {} for lab in network.nodes}
+    for x, labs in owners.items():
+        if len(labs) == 2:
+            a, b = labs
+            w = math.log2(max(ext[ids[x]], 2))
+            adj[a][b] = adj[a].get(b, 0.0) + w
+            adj[b][a] = adj[b].get(a, 0.0) + w
+
+    def size(mask):
+        s = 1
+        for bb in _bits(mask):
+            s *= ext[bb]
+        return s
+
+    def greedy(ts):
+        ts = list(ts)
+        while len(ts) > 1:
+            best = None
+            for i in range(len(ts)):
+                for j in range(i + 1, len(ts)):
+                    if ts[i].mask & ts[j].mask:
+                        c = size(ts[i].mask ^ ts[j].mask)
+                        if best is None or c < best[0]:
+                            best = (c, i, j)
+            if best is None:   # disconnected: outer product of the two smallest
+                order = sorted(range(len(ts)), key=lambda k: size(ts[k].mask))
+                best = (0, min(order[:2]), max(order[:2]))
+            _, i, j = best
+            t = _Tree(ts[i], ts[j], sizer=size)
+            ts = [x for k, x in enumerate(ts) if k not in (i, j)] + [t]
+        return ts[0]
+
+    def solve(part):
+        ts = [_Tree(label=lab, mask=leaves[lab]) for lab in part]
+        if len(ts) == 1:
+            return ts[0]
+        idx = [_bits(t.mask) for t in ts]
+        try:
+            tree, _ = optimal_tree(idx, {bb: ext[bb] for bs in idx for bb in bs}, max_work)
+        except PQError:
+            return greedy(ts)
+
+        def build(x):
+            if isinstance(x, int):
+                return ts[x]
+            return _Tree(build(x[0]), build(x[1]), sizer=size)
+        return build(tree)
+
+    def rec(part):
+        if len(part) < leaf:
+            return solve(part)
+        a, b = _bisect(part, adj, imbalance, rng)
+        if not a or not b:
+            return solve(part)
+        return _Tree(rec(a), rec(b), sizer=size)
+
+    root = rec(list(network.nodes))
+    out = []
+    counter = network.counters["node"]
+
+    def emit(t):
+        nonlocal counter
+        if t.label is not None:
+            return t.label
+        x = emit(t.left)
+        y = emit(t.right)
+        out.append([x, y])
+        counter += 1
+        return f"node_{counter}"
+    emit(root)
+    return out
+
+
 def grid_site_order(rows, cols, axis="col", open_last=()):
     """Snake order over grid sites (1-based qubits, q = r*cols + c + 1); sites
     listed in open_last go to the end (keeps open output legs out of the

@@ -112,3 +112,53 @@ def test_reconfigure_rejects_incomplete_plans():
     plan = PL.bgreedy_single(tn, 1.0, 0.0)[0]
     with pytest.raises(ValueError):
         PL.reconfigure(tn, plan[:-1])
+
+
+def test_partition_plan_with_one_leaf_is_the_exhaustive_optimum():
+    """With `leaf` above the tensor count the top-down planner solves the
+    whole network exactly (pq_netcon): the oracle's exhaustive optimum."""
+    gates = [(C.H, [0]), (C.CX, [0, 1]), (C.CX, [1, 2])]
+    tn, _ = _pair(gates, 3, "000", "0x1", decompose=False)
+    plan = PL.partition_plan(tn, leaf=64)
+    assert PL.plan_ledger(tn, plan)["macs"] == O.optimal_contraction_cost(_tensors(tn), outer=False)
+
+
+@pytest.mark.parametrize("leaf", [4, 8])
+def test_partition_plan_same_amplitudes_and_ledger(leaf):
+    """Recursive bisection down to parts of < leaf tensors on a 3 x 3 grid
+    circuit: a complete plan; the oracle contracts it to the amplitudes of a
+    greedy plan, and its ledger counts exactly the MACs the planner claims."""
+    gates = C.grid_circuit_gates(3, 3, 5, seed=11)
+    out = "01x0x1x10"
+    tp, _ = _pair(gates, 9, "0" * 9, out)
+    plan = PL.partition_plan(tp, leaf=leaf)
+    assert len(plan) == len(tp.nodes) - 1
+    results, flops = [], []
+    for p in (plan, None):
+        _, to = _pair(gates, 9, "0" * 9, out)
+        p = p if p is not None else O.greedy_plan(to)[0]
+        O.contract_network(to, p)
+        results.append(np.asarray(to.backend.load_tensor_data("result")))
+        flops.append(to.backend.metrics.flops)
+    np.testing.assert_allclose(results[0], results[1], rtol=1e-12, atol=1e-14)
+    assert flops[0] == PL.plan_ledger(tp, plan)["macs"]
+
+
+def test_bisection_is_balanced():
+    gates = C.grid_circuit_gates(3, 3, 5, seed=2)
+    tp, _ = _pair(gates, 9, "0" * 9, "0" * 9)
+    from picoquant_hip.planner import _bisect
+    nodes = list(tp.nodes)
+    adj = {v: {} for v in nodes}
+    owners = {}
+    for lab, node in tp.nodes.items():
+        for x in node.indices:
+            owners.setdefault(x, []).append(lab)
+    for labs in owners.values():
+        if len(labs) == 2:
+            a, b = labs
+            adj[a][b] = adj[a].get(b, 0) + 1.0
+            adj[b][a] = adj[b].get(a, 0) + 1.0
+    a, b = _bisect(nodes, adj, 0.1, np.random.default_rng(0))
+    assert sorted(a + b) == sorted(nodes)
+    assert abs(len(a) - len(b)) <= 0.2 * len(nodes) + 1
