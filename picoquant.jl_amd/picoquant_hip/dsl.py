@@ -14,9 +14,11 @@ text commands (one per line, a `.tl` file):
 
 execute_dsl_file replays such a file on the GPU through libpq_hip: an
 offline, Julia-free driver for a recorded contraction.  The command text is
-the reference's format; the tensor store is a numpy `.npz` archive (the
-reference writes HDF5, which has no reader in this build), one array per
-label, axis i = the tensor's index i.
+the reference's format.  The tensor store is chosen by the file's suffix:
+`.h5` / `.hdf5` is HDF5 in the layout the reference's HDF5.jl writes
+(h5store.H5TensorStore, through libhdf5), so stores recorded by the
+reference replay here; any other name is a numpy `.npz` archive, one array
+per label, axis i = the tensor's index i.
 """
 from __future__ import annotations
 
@@ -61,6 +63,14 @@ class TensorStore:
         return list(self._read())
 
 
+def open_store(path, overwrite=False):
+    """The tensor store for `path`: HDF5 for .h5 / .hdf5, else .npz."""
+    if pathlib.Path(path).suffix.lower() in (".h5", ".hdf5"):
+        from .h5store import H5TensorStore
+        return H5TensorStore(path, overwrite)
+    return TensorStore(path, overwrite)
+
+
 class DSLBackend(AbstractBackend):
     """src/backends/dsl.jl:14-44: the verbs append commands to `dsl_filename`;
     tensor data goes to the store `tensor_data_filename`; results are written
@@ -74,7 +84,7 @@ class DSLBackend(AbstractBackend):
         p = pathlib.Path(self.dsl_filename)
         if overwrite or not p.exists():
             p.write_text("")
-        self.store = TensorStore(self.tensor_data_filename, overwrite)
+        self.store = open_store(self.tensor_data_filename, overwrite)
         self.metrics = Metrics()
 
     def push(self, instruction):
@@ -90,7 +100,7 @@ class DSLBackend(AbstractBackend):
     def load_tensor_data(self, label):
         """src/backends/dsl.jl:92-106: `result` is read from the output store."""
         if str(label) == "result":
-            return TensorStore(self.output_data_filename).read(label)
+            return open_store(self.output_data_filename).read(label)
         return self.store.read(label)
 
     def contract_tensors(self, a, ia, b, ib, c):
@@ -136,7 +146,7 @@ def execute_dsl_file(dtype=np.complex64, dsl_filename="contract_network.tl", ten
     Returns the context's remaining tensor labels (handles are freed)."""
     own = ctx is None
     ctx = ctx if ctx is not None else _lib.Context(dtype, device)
-    store = TensorStore(tensor_data_filename)
+    store = open_store(tensor_data_filename)
     tensors = {}
 
     def put(label, h):
@@ -169,7 +179,7 @@ def execute_dsl_file(dtype=np.complex64, dsl_filename="contract_network.tl", ten
                 elif op == "save":
                     out = output_data_filename or cmd[2]
                     name = cmd[3] if len(cmd) > 3 else cmd[1]
-                    TensorStore(out).write(name, ctx.download(tensors[cmd[1]]))
+                    open_store(out).write(name, ctx.download(tensors[cmd[1]]))
                 elif op == "reshape":
                     h = tensors[cmd[1]]
                     shp = ctx.shape(h)

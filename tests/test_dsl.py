@@ -17,7 +17,7 @@ from picoquant_hip import circuits as C, dsl as D
 
 def oracle_replay(dsl_path, store_path):
     """The recorded program executed with numpy (the oracle as checker)."""
-    store = D.TensorStore(store_path)
+    store = D.open_store(store_path)
     t, out = {}, {}
     for line in open(dsl_path):
         cmd = line.split()
@@ -59,8 +59,9 @@ def test_record_ghz_amplitude(tmp_path):
     assert be.metrics.flops > 0
 
 
-def test_record_qft_wavefunction(tmp_path):
-    tl, h5 = tmp_path / "qft.tl", tmp_path / "qft.npz"
+@pytest.mark.parametrize("ext", ["npz", "h5"])
+def test_record_qft_wavefunction(tmp_path, ext):
+    tl, h5 = tmp_path / "qft.tl", tmp_path / f"qft.{ext}"
     be = D.DSLBackend(tl, h5, "", True)
     n, x = 5, 19
     tn = P.network_from_gates(n, C.qft_gates(n), backend=be)

@@ -216,12 +216,14 @@ def test_qasm_qft10_on_gpu(golden_dir, dtype):
     assert rel_err(psi, np.exp(2j * np.pi * x * y / 1024) / 32) < RTOL[dtype]
 
 
-def test_dsl_replay_on_gpu(tmp_path, dtype):
+@pytest.mark.parametrize("ext", ["npz", "h5"])
+def test_dsl_replay_on_gpu(tmp_path, dtype, ext):
     """A contraction recorded by DSLBackend and replayed by execute_dsl_file
     through libpq_hip: the reference's DSL test amplitude (1/sqrt 2,
-    test/layer1_tests.jl:11-44) and a QFT-6 wavefunction."""
+    test/layer1_tests.jl:11-44) and a QFT-6 wavefunction, with the tensor
+    store as .npz and as HDF5 in the reference's layout."""
     from picoquant_hip import dsl as D
-    tl, store = tmp_path / "contract_network.tl", tmp_path / "tensor_data.npz"
+    tl, store = tmp_path / "contract_network.tl", tmp_path / f"tensor_data.{ext}"
     be = D.DSLBackend(tl, store, "", True)
     tn = P.network_from_gates(3, [(C.H, [0]), (C.CX, [0, 1]), (C.CX, [1, 2])], backend=be)
     P.add_input(tn, "000")
@@ -230,7 +232,7 @@ def test_dsl_replay_on_gpu(tmp_path, dtype):
     assert D.execute_dsl_file(dtype, tl, store) == [f"node_{tn.counters['node']}"]   # the final node
     amp = complex(np.asarray(be.load_tensor_data("result")).reshape(-1)[0])
     assert abs(amp - 1 / math.sqrt(2)) < RTOL[dtype]
-    tl2, store2, out2 = tmp_path / "qft.tl", tmp_path / "qft.npz", tmp_path / "out.npz"
+    tl2, store2, out2 = tmp_path / "qft.tl", tmp_path / f"qft.{ext}", tmp_path / f"out.{ext}"
     be = D.DSLBackend(tl2, store2, out2, True)
     n, x = 6, 45
     tn = P.network_from_gates(n, C.qft_gates(n), backend=be)
