@@ -135,7 +135,10 @@ def _bgreedy_heap(tensors, counter, alpha, tau, rng):
             ix2t[x].add(c)
             nb |= ix2t[x]
         nb.discard(c)
-        for o in nb:
+        # creation order: with tau > 0 each push draws from rng, so the
+        # order must not depend on set iteration (string hashing varies
+        # between processes)
+        for o in sorted(nb, key=order.__getitem__):
             push(o, c)
         plan.append([a, b])
         tcost += math.sqrt(sa * sb * sc)

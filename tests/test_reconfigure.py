@@ -7,6 +7,8 @@ contract_network (src/layer2.jl:250-321), and the oracle's own ledger
 (Metrics.flops) counts exactly the MACs the planner claims; the cost never
 rises."""
 import math
+import os
+import pathlib
 
 import numpy as np
 import pytest
@@ -162,3 +164,29 @@ def test_bisection_is_balanced():
     a, b = _bisect(nodes, adj, 0.1, np.random.default_rng(0))
     assert sorted(a + b) == sorted(nodes)
     assert abs(len(a) - len(b)) <= 0.2 * len(nodes) + 1
+
+
+def test_sampled_greedy_plan_is_reproducible_across_processes():
+    """A Boltzmann-greedy plan (tau > 0, seeded rng) on a network large
+    enough for the heap variant is the same in processes with different
+    string-hash seeds: no draw order depends on set iteration."""
+    import subprocess
+    import sys
+    code = (
+        "import sys, hashlib, numpy as np\n"
+        "from picoquant_hip import circuits as C, planner as PL\n"
+        "import picoquant_hip as P\n"
+        "from test_host import RecordingBackend\n"
+        "tn = P.network_from_gates(30, C.grid_circuit_gates(5, 6, 12, 5), backend=RecordingBackend(), decompose=True)\n"
+        "P.add_input(tn, '0' * 30); P.add_output(tn, '0' * 30)\n"
+        "plan, tc, sc = PL.bgreedy_single(tn, tau=0.5, rng=np.random.default_rng(3), exact_limit=10)\n"
+        "print(hashlib.md5(repr(plan).encode()).hexdigest(), tc)\n")
+    root = pathlib.Path(__file__).resolve().parent.parent
+    paths = [str(root), str(root / "picoquant.jl_amd"), str(root / "tests")]
+    outs = []
+    for seed in ("1", "2"):
+        env = dict(os.environ, PYTHONHASHSEED=seed, PYTHONPATH=":".join(paths))
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=300)
+        assert r.returncode == 0, r.stderr
+        outs.append(r.stdout.strip())
+    assert outs[0] == outs[1]
