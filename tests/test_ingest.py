@@ -1,4 +1,4 @@
-"""Network ingest (SURVEY next rows): the qiskit-free QASM 2.0 parser and
+"""Network ingest: the qiskit-free QASM 2.0 parser and
 the JSON structure format of src/layer3.jl:459-707.  The QASM fixtures
 (qiskit-transpiled QFTs from the reference's examples/) must reproduce the
 discrete Fourier transform exactly, which pins gate matrices, parameter
