@@ -26,7 +26,18 @@ import sys
 import time
 
 ROOT = pathlib.Path(__file__).resolve().parent
-CPU_THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+def _host_threads():
+    """Host threads for the CPU baseline: the cores this process may run on,
+    capped by OMP_NUM_THREADS when the host sets it (a shared machine's
+    share for one GPU)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(cap))) if cap.isdigit() and int(cap) > 0 else max(1, n)
+
+
+CPU_THREADS = _host_threads()
 os.environ.setdefault("OPENBLAS_NUM_THREADS", str(CPU_THREADS))
 os.environ.setdefault("OMP_NUM_THREADS", str(CPU_THREADS))
 sys.path.insert(0, str(ROOT))
@@ -37,6 +48,8 @@ import numpy as np  # noqa: E402
 ROWS, COLS, DEPTH, SEED = 6, 6, 14, 2026
 N_SLICED = 10
 METRIC = "amplitudes/s, 36-qubit d14 grid, batched+learned vs plain GPU at ≤1e-5 rel err"
+REL_ERR_BOUND = 1e-5          # worst single amplitude against the FP64 run of the same plan
+BINDING_CAP = 2 ** 22         # per-slice size cap of the binding-slicing variant
 PEAK = {"mfma_fp32": 157.3e12, "mfma_fp64": 78.6e12, "hbm": 8.0e12}
 
 
@@ -92,9 +105,15 @@ def run_variant(tn, plan, sliced, precision, group_bits, runs=3, tune=True):
     """Autotune (for this precision), compile and time one plan variant:
     (ms per run, result vector)."""
     from picoquant_hip import autotune
-    from picoquant_hip.executor import compile_plan
+    from picoquant_hip.executor import choose_group_bits, compile_plan, lower_plan
     ctx = tn.backend.ctx
     kc = tiles = None
+    if group_bits is None:
+        # the executor's own choice (largest slice group whose batched
+        # intermediates fit its memory budget), fixed here so the autotuner
+        # times the same batch sizes
+        slots, _, _ = lower_plan(tn, plan, sliced)
+        group_bits = choose_group_bits(slots, len(sliced), 64 << 30, 8 if precision != "uniform" else 16)
     if tune and precision != "uniform":
         kc, tiles, _ = autotune.autotune_plan(ctx, tn, plan, sliced, group_bits=group_bits, precision=precision)
     p = compile_plan(tn, plan, sliced, group_bits=group_bits, kc=kc, tiles=tiles, precision=precision)
@@ -112,33 +131,103 @@ def run_variant(tn, plan, sliced, precision, group_bits, runs=3, tune=True):
 
 def grid7x7_config():
     """The 49-qubit config: 7x7 grid, 12 cycles (seed 2027), one amplitude
-    of a seeded bitstring, sweep plan with 6 sliced bonds (64 slices,
-    largest per-slice intermediate far below 2^30); FP64, complex64 on FP32
+    of a seeded bitstring, the reconfigured column sweep (the planner's
+    lowest-MAC order) with 8 bonds sliced under a per-slice cap of 2^22
+    elements -- the unsliced plan peaks at 2^24, so the cap binds and the
+    slice-batched steps carry most of the MACs; FP64, complex64 on FP32
     MFMA and FP16 hi/lo split, errors against the FP64 run."""
     import math
     import picoquant_hip as P
     from picoquant_hip import circuits as C, planner as PL
+    from picoquant_hip.executor import lower_plan
     r = c = 7
     n = r * c
-    seed = 2027
+    seed, cap, nb = 2027, 2 ** 22, 8
     tn = P.network_from_gates(n, C.grid_circuit_gates(r, c, 12, seed), backend=P.HIPBackend(np.complex128),
                               decompose=True)
     P.add_input(tn, "0" * n)
     P.add_output(tn, "".join(map(str, C.seeded_bits(n, seed + 1))))
-    plan = PL.sweep_plan(tn, PL.grid_site_order(r, c, "col"))
-    sliced, macs, peak = PL.choose_slices(tn, plan, 6, max_elems=2 ** 30)
-    out = {"workload": "7x7 grid depth-12 random circuit (seed 2027), single amplitude, 6 sliced bonds = 64 slices",
-           "sliced_macs_log2": round(math.log2(macs), 2), "peak_slice_elems_log2": round(math.log2(peak), 1)}
+    plan = PL.reconfigure(tn, PL.sweep_plan(tn, PL.grid_site_order(r, c, "col")))
+    unsliced_peak = PL.plan_ledger(tn, plan)["max_size"]
+    sliced, macs, peak = PL.choose_slices(tn, plan, nb, max_elems=cap)
+    out = {"workload": f"7x7 grid depth-12 random circuit (seed 2027), single amplitude, reconfigured column "
+                       f"sweep, {nb} bonds sliced under a per-slice cap of 2^22 = {2 ** nb} slices",
+           "sliced_macs_log2": round(math.log2(macs), 2), "peak_slice_elems_log2": round(math.log2(peak), 1),
+           "unsliced_peak_elems_log2": round(math.log2(unsliced_peak), 1),
+           "slice_dependent_mac_fraction": round(dependent_mac_fraction(tn, plan, sliced), 3)}
     ref = None
     for prec in ("uniform", "mixed", "mixed16"):
-        ms, res, cls = run_variant(tn, plan, sliced, prec, 6)
+        ms, res, cls = run_variant(tn, plan, sliced, prec, None)
         if ref is None:
             ref = res
         err = float(np.max(np.abs(res - ref) / np.abs(ref)))
         out[{"uniform": "fp64", "mixed": "fp32_mfma", "mixed16": "fp16_split"}[prec]] = {
-            "ms": round(ms, 3), "rel_err_vs_fp64": err, "classes_ms": cls}
+            "ms": round(ms, 3), "rel_err_vs_fp64": err, "within_bound": err <= REL_ERR_BOUND, "classes_ms": cls}
     out["amplitude"] = [float(ref[0].real), float(ref[0].imag)]
     return out
+
+
+def independent_bitstrings(n_bits=64, runs=3):
+    """64 independent bitstrings (every qubit closed; PCG64, seed 2028) of
+    the benchmark circuit in one run: the reconfigured column sweep of the
+    closed network, 10 sliced bonds batched inside each bitstring's group,
+    one group per bitstring, complex64 / FP64 split as the headline, one
+    hipGraph per run.  Checked against the FP64 run of the same batch and,
+    for two of the bitstrings, against single-amplitude FP64 runs."""
+    import picoquant_hip as P
+    from picoquant_hip import autotune, circuits as C, planner as PL
+    from picoquant_hip.executor import compile_plan
+    n = ROWS * COLS
+    rng = np.random.Generator(np.random.PCG64(SEED + 2))
+    batch = ["".join(str(int(b)) for b in rng.integers(0, 2, n)) for _ in range(n_bits)]
+    tn = P.network_from_gates(n, C.grid_circuit_gates(ROWS, COLS, DEPTH, SEED),
+                              backend=P.HIPBackend(np.complex128), decompose=True)
+    P.add_input(tn, "0" * n)
+    P.add_output(tn, batch[0])
+    ctx = tn.backend.ctx
+    plan = PL.reconfigure(tn, PL.sweep_plan(tn, PL.grid_site_order(ROWS, COLS, "col")))
+    sliced, macs, peak = PL.choose_slices(tn, plan, N_SLICED)
+    kc, tiles, _ = autotune.autotune_plan(ctx, tn, plan, sliced, group_bits=N_SLICED)
+    p = compile_plan(tn, plan, sliced, group_bits=N_SLICED, kc=kc, tiles=tiles, precision="mixed",
+                     bitstrings=batch)
+    t = time_plan(p, ctx, runs)
+    got = p.result().reshape(-1)
+    st = p.stats()
+    p.close()
+    p64 = compile_plan(tn, plan, sliced, group_bits=N_SLICED, bitstrings=batch)
+    p64.execute(True)
+    ref = p64.result().reshape(-1)
+    p64.close()
+    single = []
+    for b in (0, n_bits - 1):
+        t2 = P.network_from_gates(n, C.grid_circuit_gates(ROWS, COLS, DEPTH, SEED),
+                                  backend=P.HIPBackend(np.complex128), decompose=True)
+        P.add_input(t2, "0" * n)
+        P.add_output(t2, batch[b])
+        q = compile_plan(t2, plan, sliced, group_bits=N_SLICED)
+        q.execute(True)
+        single.append(abs(complex(q.result().reshape(-1)[0]) - ref[b]) / abs(ref[b]))
+        q.close()
+    return {"workload": f"{n_bits} independent bitstrings (PCG64 seed {SEED + 2}), every qubit closed, reconfigured "
+                        f"column sweep, {N_SLICED} sliced bonds, one group per bitstring, one hipGraph per run",
+            "ms_per_run": round(1e3 * t, 3), "amps_per_s": n_bits / t, "launches_per_run": st["launches"],
+            "macs_per_amplitude_log2": round(math.log2(macs), 3), "peak_slice_elems_log2": round(math.log2(peak), 1),
+            "rel_err_vs_fp64": float(np.max(np.abs(got - ref) / np.abs(ref))),
+            "fp64_batch_vs_single_runs": float(max(single))}
+
+
+def dependent_mac_fraction(tn, plan, sliced):
+    """Fraction of a sliced plan's MACs (all slices) in steps that depend on
+    a sliced bond, i.e. that run in the slice-batched part of the program."""
+    from picoquant_hip.executor import lower_plan
+    slots, steps, _ = lower_plan(tn, plan, sliced)
+    dep = tot = 0
+    for sa, sb, sc, M, N, K in steps:
+        m = math.prod(e for e, _, _ in M + N + K) << len(slots[sc].deps)
+        tot += m
+        if slots[sc].deps:
+            dep += m
+    return dep / tot
 
 
 def mps_chain_config():
@@ -179,6 +268,37 @@ def mps_chain_config():
                         "complex128", "gpu_s": round(t_gpu, 4), "cpu_oracle_s": round(t_cpu, 3),
             "speedup": round(t_cpu / t_gpu, 2), "max_bond": int(bond),
             "rel_err_vs_oracle": float(np.linalg.norm(psi - ref) / np.linalg.norm(ref))}
+
+
+def fp64_fidelity(psi, ref):
+    """|<ref|psi>|^2 / (<ref|ref><psi|psi>) of two amplitude batches, in FP64."""
+    psi = np.asarray(psi, dtype=np.complex128).reshape(-1)
+    ref = np.asarray(ref, dtype=np.complex128).reshape(-1)
+    return float(abs(np.vdot(ref, psi)) ** 2 / (np.vdot(ref, ref).real * np.vdot(psi, psi).real))
+
+
+def slicing_variants(tn, plan, ref):
+    """The benchmark plan unsliced, and sliced over 10 bonds under a per-slice
+    cap of 2^22 elements (the plan peaks at 2^26, so this cap binds and the
+    slice-batched part of the program carries most of the MACs); each in
+    the benchmark's precision, autotuned, against the FP64 result ref."""
+    import picoquant_hip.planner as PL
+    out = {}
+    for name, nb, cap in (("unsliced", 0, 2 ** 28), ("binding_cap_2^22", N_SLICED, BINDING_CAP)):
+        if nb:
+            sl, macs, peak = PL.choose_slices(tn, plan, nb, max_elems=cap)
+        else:
+            led = PL.plan_ledger(tn, plan)
+            sl, macs, peak = [], led["macs"], led["max_size"]
+        ms, res, cls = run_variant(tn, plan, sl, "mixed", None)
+        v = {"slices": 2 ** nb, "ms_per_run": round(ms, 3), "amps_per_s": 64e3 / ms,
+             "macs_log2": round(math.log2(macs), 3), "peak_slice_elems_log2": round(math.log2(peak), 1),
+             "slice_dependent_mac_fraction": round(dependent_mac_fraction(tn, plan, sl), 3) if nb else 0.0,
+             "classes_ms": cls}
+        if ref is not None:
+            v["rel_err_vs_fp64"] = float(np.max(np.abs(res - ref) / np.abs(ref)))
+        out[name] = v
+    return out
 
 
 def choose_plan(tn, open_q, model=None):
@@ -250,8 +370,14 @@ def cpu_baseline(plan, sliced, cap_s=20.0):
             break
     el = time.perf_counter() - t0
     t_slice = el * total / max(done, 1)
+    result = None
+    if steps == len(plan2):
+        # the whole slice was contracted: finish as contract_network! does
+        # (permute to the output qubits' order; untimed)
+        O.contract_network(tn, [], "vector")        # nothing left to contract: the final permute
+        result = np.asarray(tn.backend.load_tensor_data("result"), dtype=np.complex128).reshape(-1, order="F")
     return {"t_slice_s": t_slice, "steps_done": steps, "steps": len(plan2), "mac_fraction": done / total,
-            "elapsed_s": el}
+            "elapsed_s": el, "result": result}
 
 
 PROFILES = ROOT / "profiles"
@@ -312,6 +438,10 @@ def main():
     ap.add_argument("--no-fp64-check", action="store_true",
                     help="skip the FP64 reference run (and the FP16-split variant that is checked against it); "
                          "for profiler runs that should hold only the complex64 plan's kernels")
+    ap.add_argument("--no-bitstrings", action="store_true",
+                    help="skip the batch of 64 independent bitstrings")
+    ap.add_argument("--no-slicing-variants", action="store_true",
+                    help="skip the unsliced and binding-cap variants of the benchmark plan")
     ap.add_argument("--no-split16", action="store_true",
                     help="skip timing the FP16 hi/lo split variant of the benchmark plan")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -425,7 +555,7 @@ def main():
     log(f"[bench] planner: {planner_info}")
 
     # FP64 run of the same plan: the accuracy reference for the complex64 path.
-    rel_err = rel_err_norm = t_fp64_batched = None
+    rel_err = rel_err_norm = t_fp64_batched = fidelity = None
     if not args.no_fp64_check:
         p64 = compile_plan(tn, plan, sliced, group_bits=N_SLICED)
         p64.execute(True)
@@ -438,15 +568,29 @@ def main():
         p64.close()
         rel_err = float(np.max(np.abs(res64 - ref) / np.abs(ref)))
         rel_err_norm = float(np.linalg.norm(res64 - ref) / np.linalg.norm(ref))
-        log(f"[bench] max rel err vs FP64 {rel_err:.2e} (norm-wise {rel_err_norm:.2e}); "
-            f"batched FP64 run {t_fp64_batched:.3f}s")
+        fidelity = fp64_fidelity(res64, ref)
+        log(f"[bench] max rel err vs FP64 {rel_err:.2e} (norm-wise {rel_err_norm:.2e}, fidelity "
+            f"1 - {1 - fidelity:.2e}); batched FP64 run {t_fp64_batched:.3f}s")
+
+    # Slicing that binds, and no slicing at all, on the same plan.
+    slicing = None
+    if not args.no_slicing_variants:
+        slicing = slicing_variants(tn, plan, ref if not args.no_fp64_check else None)
+        log(f"[bench] slicing variants: {slicing}")
+
+    # Independent bitstrings: one contraction per bitstring, all in one run.
+    indep = None
+    if not args.no_bitstrings:
+        indep = independent_bitstrings()
+        log(f"[bench] independent bitstrings: {indep}")
 
     # The same plan with the complex64 GEMMs on FP16 hi/lo matrix cores.
     split16 = None
     if not args.no_split16 and not args.no_fp64_check:
         ms16, res16, cls16 = run_variant(tn, plan, sliced, "mixed16", N_SLICED)
+        e16 = float(np.max(np.abs(res16 - ref) / np.abs(ref)))
         split16 = {"ms_per_run": round(ms16, 3), "amps_per_s": 64e3 / ms16,
-                   "rel_err_vs_fp64": float(np.max(np.abs(res16 - ref) / np.abs(ref))),
+                   "rel_err_vs_fp64": e16, "within_bound": e16 <= REL_ERR_BOUND,
                    "rel_err_vs_fp64_normwise": float(np.linalg.norm(res16 - ref) / np.linalg.norm(ref)),
                    "classes_ms": cls16}
         log(f"[bench] FP16 hi/lo split variant: {split16}")
@@ -461,6 +605,7 @@ def main():
     plain = None
     if not args.no_plain:
         t_slice, res_slice = plain_fp64_slice_time(plan, sliced, open_q)
+        res_slice = np.asarray(res_slice).reshape(-1, order="F")
         plain = {"amps_per_s": 64 / (t_slice * 2 ** N_SLICED), "t_one_slice_s": t_slice,
                  "sample": "slice 0 of 1024 through contract_network! on HIPBackend{ComplexF64}, x1024"}
         log(f"[bench] plain FP64 path: one slice {t_slice:.2f}s -> {plain['amps_per_s']:.4f} amps/s")
@@ -474,7 +619,12 @@ def main():
                           f"{cb['steps_done']}/{cb['steps']} steps = {100 * cb['mac_fraction']:.1f}% of the "
                           f"slice's MACs in {cb['elapsed_s']:.1f}s, scaled to the slice and x1024 slices")}
         log(f"[bench] cpu baseline {cpu['value']:.3g} amps/s")
+        if plain is not None and cb["result"] is not None:
+            # the oracle finished slice 0: the plain FP64 GPU path's slice 0 against it
+            plain["slice0_rel_err_vs_oracle"] = float(np.max(np.abs(res_slice - cb["result"]) / np.abs(cb["result"])))
+            log(f"[bench] plain FP64 slice 0 vs oracle: {plain['slice0_rel_err_vs_oracle']:.2e}")
 
+    ok = rel_err is None or rel_err <= REL_ERR_BOUND
     out = {
         "metric": METRIC, "value": amps, "unit": "amplitudes/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True,
@@ -489,17 +639,25 @@ def main():
         "roofline": roof,
         "plan_roofline": plan_roof,
         "cpu_baseline": cpu,
+        "ok": ok,
+        "rel_err_bound": REL_ERR_BOUND,
         "rel_err_vs_fp64": rel_err,
         "rel_err_vs_fp64_normwise": rel_err_norm,
+        "fidelity_vs_fp64": fidelity,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain": (amps / plain["amps_per_s"]) if plain else None,
         "batched_fp64_amps_per_s": 64 / t_fp64_batched if t_fp64_batched else None,
         "planner": planner_info,
         "fp16_split_variant": split16,
+        "slicing_variants": slicing,
+        "independent_bitstrings": indep,
         "grid7x7_d12": grid7,
         "mps_chain20": mps,
     }
     print(json.dumps(out))
+    if not ok:
+        log(f"[bench] FAILED: worst amplitude error {rel_err:.3g} exceeds the {REL_ERR_BOUND:g} bound")
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -49,6 +49,15 @@ int pq_synchronize(pq_ctx* ctx);
 int pq_tensor_upload(pq_ctx* ctx, int rank, const int64_t* dims, const void* host,
                      int host_dtype, pq_tensor* out);
 
+/* Overwrite a contiguous tensor's elements in place with host data of the
+ * same shape (host_dtype converted to the context's type); stream-ordered
+ * after earlier work on the context, synchronising.  Compiled plans and
+ * captured graphs that read the tensor see the new values on their next
+ * run: this is how a plan's output bitstrings change without recompiling.
+ * The reference's save_tensor_data on an existing label
+ * (src/backends/interactive.jl:32-36) stores a new array instead. */
+int pq_tensor_write(pq_ctx* ctx, pq_tensor t, const void* host, int host_dtype);
+
 /* Shape query with no data transfer (partition_network_on_virtual_bonds reads
  * only the shape, src/layer2/slicing.jl:46-48). dims must hold PQ_MAX_RANK. */
 int pq_tensor_shape(pq_ctx* ctx, pq_tensor t, int* rank, int64_t* dims);
@@ -60,7 +69,7 @@ int pq_tensor_download(pq_ctx* ctx, pq_tensor t, void* host);
 /* delete_tensor! (src/backends/interactive.jl:159-161). */
 int pq_tensor_free(pq_ctx* ctx, pq_tensor t);
 
-/* contract_tensors (src/backends/interactive.jl:60-75 -> src/layer1.jl:323-330):
+/* contract_tensors (src/backends/interactive.jl:60-75 -> src/layer1.jl:85-92):
  * ncon labels, positive = summed (each appears once in A and once in B),
  * negative = open; C's index order is symdiff(ia, ib): A's open indices in A's
  * order, then B's open indices in B's order.  A and B are not consumed; the
@@ -68,7 +77,7 @@ int pq_tensor_free(pq_ctx* ctx, pq_tensor t);
 int pq_contract(pq_ctx* ctx, pq_tensor A, int rankA, const int32_t* ia,
                 pq_tensor B, int rankB, const int32_t* ib, pq_tensor* C);
 
-/* permute_tensor (src/backends/interactive.jl:111-115 -> src/layer1.jl:349-352):
+/* permute_tensor (src/backends/interactive.jl:111-115 -> src/layer1.jl:111-114):
  * perm is 1-based as in permutedims.  Zero-copy: the result is a strided view
  * that consumers read through; download/reshape materialise it. */
 int pq_permute(pq_ctx* ctx, pq_tensor t, const int32_t* perm, pq_tensor* out);
@@ -78,7 +87,7 @@ int pq_permute(pq_ctx* ctx, pq_tensor t, const int32_t* perm, pq_tensor* out);
 int pq_reshape(pq_ctx* ctx, pq_tensor t, int rank, const int64_t* dims);
 
 /* view_tensor! -> tensor_view (src/backends/interactive.jl:169-172,
- * src/layer1.jl:429-432): restrict 1-based axis to lo:hi (inclusive); the
+ * src/layer1.jl:191-194): restrict 1-based axis to lo:hi (inclusive); the
  * axis is kept with extent hi-lo+1.  Zero-copy. */
 int pq_view(pq_ctx* ctx, pq_tensor t, int axis, int64_t lo, int64_t hi, pq_tensor* out);
 
@@ -199,6 +208,13 @@ typedef struct {
                                        chain_words (NULL when no step has kind 2) */
   int64_t n_chain_words;
   const int32_t* chain_words;       /* fused-chain programs (PQ_CHAIN_* below) */
+  int32_t n_out_bits;               /* the last n_out_bits sliced bonds (all above the slice group)
+                                       number output batches instead of slices: the result holds
+                                       2^n_out_bits blocks of the output slot's elements, block b
+                                       summed over the slices whose values of those bonds spell b
+                                       (bond n_sliced - n_out_bits + k = bit k of b).  A batch of
+                                       bitstrings: each closed output qubit is a leaf with one
+                                       column per bitstring, read through these bonds' strides */
 } pq_plan_desc;
 
 /* A fused chain applies L small complex64 operators one after the other to
@@ -230,11 +246,16 @@ typedef struct {
  * many bytes. */
 #define PQ_CHAIN_STAGE_BYTES 16384
 
+/* The plan holds a reference on every leaf's device buffer and reads the
+ * leaf at the offset and strides it had here: freeing or reshaping a leaf
+ * tensor afterwards does not affect the plan.  Destroy plans before their
+ * context. */
 int pq_plan_create(pq_ctx* ctx, const pq_plan_desc* desc, pq_plan** out);
 /* Enqueue one full run (all slices + FP64 accumulation); use_graph=1 captures
  * the run into a hipGraph on first use and replays it afterwards. */
 int pq_plan_execute(pq_ctx* ctx, pq_plan* plan, int use_graph);
-/* Copy the FP64 result (2 * out_elems doubles, interleaved) to the host. */
+/* Copy the FP64 result (2 * out_elems doubles, interleaved; out_elems as
+ * pq_plan_stats reports it, all output batches) to the host. */
 int pq_plan_result(pq_ctx* ctx, pq_plan* plan, double* host);
 int pq_plan_stats(const pq_plan* plan, int64_t* arena_bytes, int64_t* launches_per_run,
                   double* alg_flops, double* alg_bytes, int64_t* out_elems);

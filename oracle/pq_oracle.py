@@ -110,7 +110,7 @@ def record_compute_costs(backend, open_dims, contracted_dims):
 
 
 def contract_tensors_data(A, ia, B, ib):
-    """src/layer1.jl:323-330: tensorcontract(A, ia, B, ib, symdiff(ia, ib)).
+    """src/layer1.jl:85-92: tensorcontract(A, ia, B, ib, symdiff(ia, ib)).
 
     Positive labels are summed, negative labels are open; the output order is
     symdiff(ia, ib) = A's open labels in A order, then B's open labels in B order.
@@ -150,12 +150,12 @@ class InteractiveBackend:
     def save_output(self, node, name="result"):        # :84-88
         self.save_tensor_data(name, self.tensors[node])
 
-    def reshape_tensor(self, label, groups):           # :97-102, src/layer1.jl:338-341
+    def reshape_tensor(self, label, groups):           # :97-102, src/layer1.jl:100-103
         t = self.tensors[label]
         dims = [int(np.prod([t.shape[g - 1] for g in grp])) for grp in groups]
         self.tensors[label] = np.reshape(t, dims, order="F")
 
-    def permute_tensor(self, label, axes):             # :111-115, src/layer1.jl:349-352
+    def permute_tensor(self, label, axes):             # :111-115, src/layer1.jl:111-114
         self.tensors[label] = np.transpose(self.tensors[label], [a - 1 for a in axes])
 
     def delete_tensor(self, label):                    # :159-161
@@ -166,11 +166,11 @@ class InteractiveBackend:
         sl = [slice(None)] * t.ndim
         lo, hi = bond_range
         sl[bond_idx - 1] = slice(lo - 1, hi)          # Julia lo:hi, 1-based inclusive
-        self.tensors[view_node] = np.array(t[tuple(sl)])   # src/layer1.jl:429-432
+        self.tensors[view_node] = np.array(t[tuple(sl)])   # src/layer1.jl:191-194
 
     def decompose_tensor(self, label, left, right, threshold=1e-13, max_rank=0,
                          left_label=None, right_label=None):
-        """src/backends/interactive.jl:130-152 -> src/layer1.jl:384-422."""
+        """src/backends/interactive.jl:130-152 -> src/layer1.jl:146-184."""
         t = self.tensors[label]
         B, C, chi = decompose_tensor_data(t, left, right, threshold, max_rank)
         self.tensors[left_label] = B
@@ -180,7 +180,7 @@ class InteractiveBackend:
 
 
 def decompose_tensor_data(t, left, right, threshold=1e-13, max_rank=0):
-    """src/layer1.jl:384-422 (SVD split; chi from the normalised spectrum)."""
+    """src/layer1.jl:146-184 (SVD split; chi from the normalised spectrum)."""
     dims = t.shape
     ld = [dims[x - 1] for x in left]
     rd = [dims[x - 1] for x in right]

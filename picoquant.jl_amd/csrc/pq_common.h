@@ -2,7 +2,7 @@
 // division by run-time invariant extents, and the index-group descriptors a
 // pairwise contraction is lowered to.
 //
-// A pairwise contraction C = sum_K A * B (src/layer1.jl:323-330) is described
+// A pairwise contraction C = sum_K A * B (src/layer1.jl:85-92) is described
 // by three index groups over column-major storage:
 //   M : A's open indices   (strides in A and in C)
 //   N : B's open indices   (strides in B and in C)

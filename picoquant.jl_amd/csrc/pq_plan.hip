@@ -92,6 +92,9 @@ struct Slot {
   int64_t copies() const { return (int64_t)1 << __builtin_popcountll(low); }
   int dtype = PQ_C128;
   pq_tensor leaf = 0;
+  Buf* leaf_buf = nullptr;            // leaves: the buffer, referenced by the plan until it is destroyed
+  int64_t leaf_off = 0;               // leaves: element offset of the tensor in leaf_buf
+  int64_t leaf_span = 0;              // leaves: largest element offset the tensor's view reaches
   std::vector<int64_t> slice_stride;  // unbatched slots: stride of each sliced bond kept (0 = absent)
   int64_t arena_off = -1;             // intermediates (bytes)
   int producer = -1;
@@ -132,6 +135,8 @@ struct pq_plan {
   int32_t* chain_tab = nullptr;      // device copy of the fused-chain address tables
   int n_sliced = 0, group_bits = 0;
   int64_t n_groups = 1;
+  int n_out_bits = 0;                // the top n_out_bits sliced bonds number output batches
+  int64_t n_out = 1;                 // 2^n_out_bits results of out_elems each
   int out_slot = -1;
   void* arena = nullptr;
   size_t arena_bytes = 0;
@@ -156,10 +161,7 @@ void* slot_ptr(pq_ctx* c, pq_plan* p, int s, int64_t group) {
   int64_t off = 0;
   for (int j = p->group_bits; j < p->n_sliced; ++j)
     if ((group >> (j - p->group_bits)) & 1) off += sl.slice_stride[j];
-  if (sl.leaf) {
-    Tensor* t = get(c, sl.leaf);
-    return (unsigned char*)t->buf->ptr + (t->off + off) * c->esize;
-  }
+  if (sl.leaf) return (unsigned char*)sl.leaf_buf->ptr + (sl.leaf_off + off) * c->esize;
   return (unsigned char*)p->arena + sl.arena_off + off * (int64_t)esize_of(sl.dtype);
 }
 
@@ -169,7 +171,11 @@ int64_t span_of(const std::vector<Mode>& ms, bool first) {
   return s;
 }
 
-void free_plan(pq_plan* p) {
+// Frees the plan's device memory and releases its references on leaf
+// buffers (returned to the context's pool when nothing else holds them).
+void free_plan(pq_ctx* c, pq_plan* p) {
+  for (Slot& sl : p->slots)
+    if (sl.leaf_buf) unref(c, sl.leaf_buf);
   if (p->arena) hipFree(p->arena);
   if (p->scratch) hipFree(p->scratch);
   if (p->out_dev) hipFree(p->out_dev);
@@ -179,12 +185,9 @@ void free_plan(pq_plan* p) {
 }
 
 // Whether `span` (largest element offset read or written) stays inside slot s.
-bool slot_holds(pq_ctx* c, const Slot& sl, int64_t span) {
+bool slot_holds(const Slot& sl, int64_t span) {
   if (!sl.leaf) return span < sl.elems;
-  Tensor* t = get(c, sl.leaf);
-  int64_t full = 0;
-  for (int q = 0; q < t->rank; ++q) full += (t->dims[q] - 1) * t->strides[q];
-  return span <= full;
+  return span <= sl.leaf_span;
 }
 
 // Appends the address table of a bit list (bits [b0, b1)) to tab: entry t
@@ -330,13 +333,17 @@ const char* parse_chain(pq_ctx* c, pq_plan* p, const pq_plan_desc* d, int i, PSt
   }
   if (!ok || !fill_group(a.spec, spec)) return "bad spectator modes";
   if (X.dtype != PQ_C64 || Y.dtype != PQ_C64) return "fused chain needs complex64 tensors";
-  if (X.mask || Y.mask) return "fused chain on a slice-dependent tensor";
+  // A chain runs once per slice group at most: its tensors may depend on
+  // bonds above the group (the step then reruns per group, at the group's
+  // addresses) but not be batched over the slices of a group.
+  if (X.low || Y.low) return "fused chain on a tensor batched over the slices of a group";
+  if (X.mask & ~Y.mask) return "fused chain output independent of a bond its input depends on";
   for (int64_t v : X.slice_stride)
     if (v) return "fused chain input keeps a sliced bond";
   for (int64_t v : Y.slice_stride)
     if (v) return "fused chain output keeps a sliced bond";
-  if (!slot_holds(c, X, xspan)) return "gather out of range";
-  if (!slot_holds(c, Y, yspan)) return "scatter out of range";
+  if (!slot_holds(X, xspan)) return "gather out of range";
+  if (!slot_holds(Y, yspan)) return "scatter out of range";
   if (((int64_t)a.spec.total << nout) != Y.elems) return "scatter does not cover the output";
   // Each side moves element pairs along one bit (the stride-1 bit if there
   // is one, else the smallest stride; none for a side without bits: the
@@ -388,9 +395,17 @@ const char* parse_chain(pq_ctx* c, pq_plan* p, const pq_plan_desc* d, int i, PSt
       return "bad fused-chain stage";
     const Slot& S = p->slots[os];
     if (!S.leaf && produced[os] < 0) return "operator read before it is produced";
-    if (S.dtype != PQ_C64 || S.mask) return "fused-chain operator must be an unbatched complex64 tensor";
-    for (int64_t v : S.slice_stride)
-      if (v) return "fused-chain operator keeps a sliced bond";
+    if (S.dtype != PQ_C64 || S.low) return "fused-chain operator must be a complex64 tensor not batched in a group";
+    if (S.mask & ~Y.mask) return "fused chain output independent of a bond its operator depends on";
+    // bonds above the group that the operator keeps as indices: read at the
+    // group's offset (every group's offset checked below)
+    int64_t shigh = 0;
+    for (int j = 0; j < p->n_sliced; ++j) {
+      if (!S.slice_stride[j]) continue;
+      if (j < p->group_bits) return "fused-chain operator keeps a sliced bond of the slice group";
+      if (!((Y.mask >> j) & 1)) return "fused chain output independent of a bond its operator keeps";
+      shigh += S.slice_stride[j];
+    }
     std::vector<int32_t> kpos, kstr, mpos, mstr, rpos;
     int64_t sspan = 0;
     uint32_t kr = 0, mr = 0, rr = 0;
@@ -400,7 +415,7 @@ const char* parse_chain(pq_ctx* c, pq_plan* p, const pq_plan_desc* d, int i, PSt
     // R must be disjoint from K and from M: every (r, k) and (r, m) address
     // is then distinct across r, which the in-place update relies on
     if ((rr & kr) || (rr & mr)) return "R bits overlap K or M bits";
-    if (!slot_holds(c, S, sspan)) return "operator access out of range";
+    if (!slot_holds(S, sspan + shigh)) return "operator access out of range";
     g.nk = nk;
     g.nm = nm;
     g.rlo = std::min(nr, 4);
@@ -462,13 +477,17 @@ const char* parse_chain(pq_ctx* c, pq_plan* p, const pq_plan_desc* d, int i, PSt
   st.bytes += 8.0 * (double)a.spec.total * (double)(((int64_t)1 << nin) + ((int64_t)1 << nout));
   st.spec.kc = PQ_KC_CHAIN;
   st.spec.tile = -1;
-  st.dep = false;
-  st.phase2 = false;
+  st.dep = Y.batched;
+  st.phase2 = (Y.mask >> p->group_bits) != 0;
+  if (st.phase2) {
+    st.flops *= (double)p->n_groups;
+    st.bytes *= (double)p->n_groups;
+  }
   return nullptr;
 }
 
 int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
-  int64_t nb = p->out_elems;
+  int64_t nb = p->out_elems * p->n_out;
   unsigned g = (unsigned)std::min<int64_t>((nb + 255) / 256, 1024);
   hipLaunchKernelGGL(k_fill_zero, dim3(std::max(1u, g)), dim3(256), 0, c->stream, p->out_dev, 2 * nb);
   if (hipGetLastError() != hipSuccess) return fail(c, PQ_ERR_HIP, "fill kernel");
@@ -553,13 +572,15 @@ int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
       if (r) return r;
     }
     const void* src = slot_ptr(c, p, p->out_slot, grp);
+    // the group's output batch: its bits above the summed slice bonds
+    double* dst = p->out_dev + 2 * p->out_elems * (grp >> (p->n_sliced - p->n_out_bits - p->group_bits));
     const unsigned ga = (unsigned)((os.elems + 15) / 16);
     if (os.dtype == PQ_C128)
       hipLaunchKernelGGL((k_accum<c128>), dim3(ga), dim3(256), 0, c->stream, (const c128*)src,
-                         os.elems, os.elems, nbatch, p->out_dev);
+                         os.elems, os.elems, nbatch, dst);
     else
       hipLaunchKernelGGL((k_accum<c64>), dim3(ga), dim3(256), 0, c->stream, (const c64*)src,
-                         os.elems, os.elems, nbatch, p->out_dev);
+                         os.elems, os.elems, nbatch, dst);
     if (hipGetLastError() != hipSuccess) return fail(c, PQ_ERR_HIP, "accumulate kernel");
   }
   return PQ_OK;
@@ -578,9 +599,15 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
   p->n_sliced = d->n_sliced;
   p->group_bits = d->group_bits;
   p->n_groups = (int64_t)1 << (d->n_sliced - d->group_bits);
+  if (d->n_out_bits < 0 || d->n_out_bits > d->n_sliced - d->group_bits) {
+    delete p;
+    return fail(c, PQ_ERR_ARG, "output batch bonds must lie above the slice group");
+  }
+  p->n_out_bits = d->n_out_bits;
+  p->n_out = (int64_t)1 << d->n_out_bits;
   p->out_slot = d->out_slot;
   auto bad = [&](const std::string& m) {
-    free_plan(p);
+    free_plan(c, p);
     return fail(c, PQ_ERR_ARG, m);
   };
   p->slots.resize(d->n_slots);
@@ -613,6 +640,12 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
       int64_t span = 0;
       for (int i = 0; i < t->rank; ++i) span += (t->dims[i] - 1) * t->strides[i];
       if (span + 1 > (int64_t)(t->buf->bytes / c->esize) - t->off) return bad("leaf view exceeds its buffer");
+      // the plan reads this buffer at this offset on every run: hold it, so
+      // that freeing or reshaping the tensor later leaves the plan intact
+      sl.leaf_buf = t->buf;
+      sl.leaf_buf->refs++;
+      sl.leaf_off = t->off;
+      sl.leaf_span = span;
       if (extra > span) return bad("slice strides exceed the leaf");
     } else if (extra >= sl.elems) {
       return bad("slice strides exceed the slot");
@@ -665,7 +698,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
       if (why) return bad(std::string("step ") + std::to_string(i) + ": " + why);
       p->alg_flops += st.flops;
       p->alg_bytes += st.bytes;
-      p->order1.push_back(i);
+      (st.phase2 ? p->order2 : p->order1).push_back(i);
       continue;
     }
     if (st.kind == 1) {
@@ -686,10 +719,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
       for (int j = 0; j < d->n_sliced; ++j)
         if ((Cs.mask >> j) & 1) extra += sl.slice_stride[j];
       if (!sl.leaf) return span + extra < sl.elems;
-      Tensor* t = get(c, sl.leaf);
-      int64_t full = 0;
-      for (int q = 0; q < t->rank; ++q) full += (t->dims[q] - 1) * t->strides[q];
-      return span + extra <= full;
+      return span + extra <= sl.leaf_span;
     };
     if (!fits(A, span_of(st.spec.M, true) + span_of(st.spec.K, true))) return bad("A access out of range");
     if (!fits(B, span_of(st.spec.N, true) + span_of(st.spec.K, false))) return bad("B access out of range");
@@ -725,12 +755,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
         offC += L.sC[q];
       }
       auto inside = [&](const Slot& sl, int64_t off, int64_t span) -> bool {
-        if (sl.leaf) {
-          Tensor* t = get(c, sl.leaf);
-          int64_t full = 0;
-          for (int q = 0; q < t->rank; ++q) full += (t->dims[q] - 1) * t->strides[q];
-          return off + span <= full;
-        }
+        if (sl.leaf) return off + span <= sl.leaf_span;
         return off + span < sl.elems * sl.copies();
       };
       if (!inside(A, offA, span_of(st.spec.M, true) + span_of(st.spec.K, true)) ||
@@ -766,7 +791,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     int r = resolve_step(c, st.spec, esize_of(st.spec.dtype));
     if (r) {
       std::string msg = c->err;
-      free_plan(p);
+      free_plan(c, p);
       return fail(c, r, "step " + std::to_string(i) + ": " + msg);
     }
     const double nbat = (double)st.spec.L.total * (st.phase2 ? (double)p->n_groups : 1.0);
@@ -865,8 +890,10 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     int stop = sl.last_use >= 0 ? pos[sl.last_use] : start;
     if (s == p->out_slot) stop = end;
     bool read_in_2 = false;
-    for (int i : p->order2)
-      if (p->steps[i].a == s || p->steps[i].b == s) read_in_2 = true;
+    for (int i : p->order2) {
+      const PStep& q = p->steps[i];
+      if (q.a == s || q.b == s || std::find(q.ops.begin(), q.ops.end(), s) != q.ops.end()) read_in_2 = true;
+    }
     if (read_in_2 && start < phase2_start) stop = end;
     size_t bytes = (size_t)sl.elems * esize_of(sl.dtype) * (size_t)sl.copies();
     bytes = (bytes + 255) / 256 * 256;
@@ -895,9 +922,9 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
   p->out_elems = p->slots[p->out_slot].elems;
   if (hipMalloc(&p->arena, p->arena_bytes) != hipSuccess ||
       (p->scratch_bytes && hipMalloc(&p->scratch, p->scratch_bytes) != hipSuccess) ||
-      hipMalloc((void**)&p->out_dev, sizeof(double) * 2 * p->out_elems) != hipSuccess) {
+      hipMalloc((void**)&p->out_dev, sizeof(double) * 2 * p->out_elems * p->n_out) != hipSuccess) {
     (void)hipGetLastError();
-    free_plan(p);
+    free_plan(c, p);
     return fail(c, PQ_ERR_OOM, "plan memory allocation failed");
   }
   // Grouped descriptors (pointers of slice group 0 + per-bit leaf offsets).
@@ -906,7 +933,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     for (auto* seq : {&p->seq1, &p->seq2})
       for (const Launch& L : *seq) any = any || L.grouped;
     if (any) {
-      free_plan(p);
+      free_plan(c, p);
       return fail(c, PQ_ERR_UNSUPPORTED, "too many slice groups for grouped launches");
     }
   }
@@ -929,14 +956,14 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
           const int64_t n = conv_elems(st);
           m.conv = 1;
           if (!fill_group(m.M, {{n, 1, 1}}) || !fill_group(m.N, {}) || !fill_group(m.K, {})) {
-            free_plan(p);
+            free_plan(c, p);
             return fail(c, PQ_ERR_UNSUPPORTED, "grouped conversion too large");
           }
           m.L = single_batch();
           maxw = std::max<uint64_t>(maxw, (uint64_t)n);
         } else {
           if (!fill_group(m.M, st.spec.M) || !fill_group(m.N, st.spec.N) || !fill_group(m.K, st.spec.K)) {
-            free_plan(p);
+            free_plan(c, p);
             return fail(c, PQ_ERR_UNSUPPORTED, "too many modes in a grouped step");
           }
           m.L = st.spec.L;
@@ -961,7 +988,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     if (hipMalloc((void**)&p->descs, hd.size() * sizeof(MultiDesc)) != hipSuccess ||
         hipMemcpy(p->descs, hd.data(), hd.size() * sizeof(MultiDesc), hipMemcpyHostToDevice) != hipSuccess) {
       (void)hipGetLastError();
-      free_plan(p);
+      free_plan(c, p);
       return fail(c, PQ_ERR_OOM, "descriptor upload failed");
     }
   }
@@ -970,7 +997,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
         hipMemcpy(p->chain_tab, chain_tab.data(), chain_tab.size() * sizeof(int32_t), hipMemcpyHostToDevice) !=
             hipSuccess) {
       (void)hipGetLastError();
-      free_plan(p);
+      free_plan(c, p);
       return fail(c, PQ_ERR_OOM, "fused-chain table upload failed");
     }
   }
@@ -1006,7 +1033,8 @@ int pq_plan_execute(pq_ctx* c, pq_plan* p, int use_graph) {
 
 int pq_plan_result(pq_ctx* c, pq_plan* p, double* host) {
   if (!c || !p || !host) return PQ_ERR_ARG;
-  hipError_t e = hipMemcpyAsync(host, p->out_dev, sizeof(double) * 2 * p->out_elems, hipMemcpyDeviceToHost,
+  hipError_t e = hipMemcpyAsync(host, p->out_dev, sizeof(double) * 2 * p->out_elems * p->n_out,
+                                hipMemcpyDeviceToHost,
                                 c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hipfail(c, e, "plan result");
@@ -1020,7 +1048,7 @@ int pq_plan_stats(const pq_plan* p, int64_t* arena_bytes, int64_t* launches, dou
   if (launches) *launches = p->launches;
   if (alg_flops) *alg_flops = p->alg_flops;
   if (alg_bytes) *alg_bytes = p->alg_bytes;
-  if (out_elems) *out_elems = p->out_elems;
+  if (out_elems) *out_elems = p->out_elems * p->n_out;
   return PQ_OK;
 }
 
@@ -1064,18 +1092,20 @@ int pq_plan_profile(pq_ctx* c, pq_plan* p, double* step_ms) {
     float ms = 0;
     e = hipEventElapsedTime(&ms, p->ev[2 * i], p->ev[2 * i + 1]);
     if (e != hipSuccess) return hipfail(c, e, "profile event");
-    step_ms[i] = p->steps[i].dep ? ms * (double)ng : ms;
+    // phase-2 launches (those of seq2, grouped or not) run once per slice
+    // group; phase-1 launches once, batched over a group's slices or not
+    step_ms[i] = p->steps[i].phase2 ? ms * (double)ng : ms;
   }
   return PQ_OK;
 }
 
 int pq_plan_destroy(pq_ctx* c, pq_plan* p) {
-  if (!p) return PQ_ERR_ARG;
-  if (c) hipStreamSynchronize(c->stream);
+  if (!c || !p) return PQ_ERR_ARG;
+  hipStreamSynchronize(c->stream);
   if (p->exec) hipGraphExecDestroy(p->exec);
   if (p->graph) hipGraphDestroy(p->graph);
   for (auto& e : p->ev) hipEventDestroy(e);
-  free_plan(p);
+  free_plan(c, p);
   return PQ_OK;
 }
 

@@ -516,6 +516,35 @@ int pq_synchronize(pq_ctx* c) {
   return PQ_OK;
 }
 
+namespace {
+
+// Stream-ordered copy of n host elements (host_dtype) into device memory of
+// the context's element type, converting on the host when the types differ;
+// synchronises, so the host buffer may be reused on return.
+hipError_t host_to_device(pq_ctx* c, void* dst, const void* host, int host_dtype, int64_t n) {
+  if (n <= 0) return hipSuccess;
+  std::vector<unsigned char> tmp;
+  const void* src = host;
+  if (host_dtype != c->dtype) {
+    tmp.resize((size_t)n * c->esize);
+    if (c->dtype == PQ_C64) {
+      const double* s = (const double*)host;
+      float* d = (float*)tmp.data();
+      for (int64_t i = 0; i < 2 * n; ++i) d[i] = (float)s[i];
+    } else {
+      const float* s = (const float*)host;
+      double* d = (double*)tmp.data();
+      for (int64_t i = 0; i < 2 * n; ++i) d[i] = (double)s[i];
+    }
+    src = tmp.data();
+  }
+  hipError_t e = hipMemcpyAsync(dst, src, (size_t)n * c->esize, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return e;
+}
+
+}  // namespace
+
 int pq_tensor_upload(pq_ctx* c, int rank, const int64_t* dims, const void* host, int host_dtype,
                      pq_tensor* out) {
   if (!c || !out || rank < 0 || rank > PQ_MAX_RANK || (rank > 0 && !dims))
@@ -533,30 +562,25 @@ int pq_tensor_upload(pq_ctx* c, int rank, const int64_t* dims, const void* host,
   if (n > 0 && !host) return fail(c, PQ_ERR_ARG, "null host pointer");
   t.buf = new_buf(c, (size_t)n * c->esize);
   if (!t.buf) return fail(c, PQ_ERR_OOM, "device allocation failed");
-  if (n > 0) {
-    std::vector<unsigned char> tmp;
-    const void* src = host;
-    if (host_dtype != c->dtype) {
-      tmp.resize((size_t)n * c->esize);
-      if (c->dtype == PQ_C64) {
-        const double* s = (const double*)host;
-        float* d = (float*)tmp.data();
-        for (int64_t i = 0; i < 2 * n; ++i) d[i] = (float)s[i];
-      } else {
-        const float* s = (const float*)host;
-        double* d = (double*)tmp.data();
-        for (int64_t i = 0; i < 2 * n; ++i) d[i] = (double)s[i];
-      }
-      src = tmp.data();
-    }
-    hipError_t e = hipMemcpyAsync(t.buf->ptr, src, (size_t)n * c->esize, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) {
-      unref(c, t.buf);
-      return hipfail(c, e, "upload");
-    }
+  hipError_t e = host_to_device(c, t.buf->ptr, host, host_dtype, n);
+  if (e != hipSuccess) {
+    unref(c, t.buf);
+    return hipfail(c, e, "upload");
   }
   *out = add(c, t);
+  return PQ_OK;
+}
+
+int pq_tensor_write(pq_ctx* c, pq_tensor h, const void* host, int host_dtype) {
+  if (!c) return PQ_ERR_ARG;
+  if (host_dtype != PQ_C128 && host_dtype != PQ_C64) return fail(c, PQ_ERR_ARG, "bad host dtype");
+  Tensor* t = get(c, h);
+  if (!t) return fail(c, PQ_ERR_NOT_FOUND, "unknown tensor handle");
+  if (!is_contiguous(*t)) return fail(c, PQ_ERR_ARG, "tensor_write needs a contiguous tensor");
+  const int64_t n = numel(*t);
+  if (n > 0 && !host) return fail(c, PQ_ERR_ARG, "null host pointer");
+  hipError_t e = host_to_device(c, (unsigned char*)t->buf->ptr + t->off * c->esize, host, host_dtype, n);
+  if (e != hipSuccess) return hipfail(c, e, "tensor write");
   return PQ_OK;
 }
 

@@ -50,6 +50,7 @@ SIGNATURES = {
     "pq_tensor_shape": (ct.c_int, [_P, ct.c_int64, ct.POINTER(ct.c_int), _I64P]),
     "pq_tensor_download": (ct.c_int, [_P, ct.c_int64, _P]),
     "pq_tensor_free": (ct.c_int, [_P, ct.c_int64]),
+    "pq_tensor_write": (ct.c_int, [_P, ct.c_int64, _P, ct.c_int]),
     "pq_contract": (ct.c_int, [_P, ct.c_int64, ct.c_int, _I32P, ct.c_int64, ct.c_int, _I32P, _I64P]),
     "pq_permute": (ct.c_int, [_P, ct.c_int64, _I32P, _I64P]),
     "pq_decompose": (ct.c_int, [_P, ct.c_int64, ct.c_int, _I32P, ct.c_int, _I32P, ct.c_double, ct.c_int,
@@ -189,6 +190,17 @@ class Context:
         out = np.empty(int(np.prod(shp)) if shp else 1, dtype=self.np_dtype)
         self.check(self.lib.pq_tensor_download(self.ptr, h, out.ctypes.data_as(ct.c_void_p)))
         return out.reshape(shp, order="F")
+
+    def write(self, h, data):
+        """Overwrite tensor h's elements in place (pq_tensor_write)."""
+        a = np.asarray(data)
+        if tuple(a.shape) != self.shape(h):
+            raise ValueError(f"shape {a.shape} does not match the tensor's {self.shape(h)}")
+        if a.dtype == np.complex64:
+            host, code = np.asfortranarray(a), PQ_C64
+        else:
+            host, code = np.asfortranarray(a.astype(np.complex128, copy=False)), PQ_C128
+        self.check(self.lib.pq_tensor_write(self.ptr, h, host.ctypes.data_as(ct.c_void_p), code))
 
     def free(self, h):
         self.check(self.lib.pq_tensor_free(self.ptr, h))

@@ -49,6 +49,7 @@ class PlanDesc(ct.Structure):
         ("step_chain", ct.POINTER(ct.c_int64)),
         ("n_chain_words", ct.c_int64),
         ("chain_words", ct.POINTER(ct.c_int32)),
+        ("n_out_bits", ct.c_int32),
     ]
 
 
@@ -96,9 +97,15 @@ def _assign_strides(C):
 
 
 def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf_strides=None,
-               carry_elems=SMALL_ELEMS):
+               carry_elems=SMALL_ELEMS, extra_bonds=None):
     """Pure-host lowering (no device calls): returns (slots, steps, out_slot)
     where steps = [(a, b, c, M, N, K)] with modes as (ext, s0, s1).
+
+    extra_bonds: {leaf label: [stride of bond k, k = 0..n-1]} -- n more
+    extent-2 bonds after the sliced ones that are no edge of the network
+    but that these leaves keep as indices at the given strides (a batch of
+    output bitstrings: each closed output qubit's leaf holds one column per
+    bitstring, and bond k selects bit k of the bitstring's number).
 
     carry_elems: an intermediate whose operands are slice-independent keeps
     their sliced bonds as indices (runs once, for every value of those
@@ -107,7 +114,9 @@ def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf
     touches a sliced bond is then batched over the slices)."""
     sliced = list(sliced)
     sset = set(sliced)
-    nsl = len(sliced)
+    extra_bonds = extra_bonds or {}
+    n_extra = max((len(v) for v in extra_bonds.values()), default=0)
+    nsl = len(sliced) + n_extra
     slots: list[SlotInfo] = []
     label_slot = {}
     for lab, node in network.nodes.items():
@@ -118,7 +127,9 @@ def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf
         else:
             strides = [int(np.prod(dims[:i])) for i in range(len(dims))]
         s.leaf = leaf_handles[lab] if leaf_handles is not None else 0
-        s.slice_stride = [0] * len(sliced)
+        s.slice_stride = [0] * nsl
+        for k, st in enumerate(extra_bonds.get(lab, ())):
+            s.slice_stride[len(sliced) + k] = st
         for ix, d, st in zip(node.indices, dims, strides):
             if ix in sset:
                 if d != 2:
@@ -128,7 +139,7 @@ def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf
                 s.idx.append(ix)
                 s.ext[ix] = d
                 s.stride[ix] = st
-        s.elems = int(np.prod(dims)) if dims else 1
+        s.elems = (int(np.prod(dims)) if dims else 1) << len(extra_bonds.get(lab, ()))
         label_slot[lab] = len(slots)
         slots.append(s)
     n_leaves = len(slots)
@@ -403,24 +414,47 @@ def chain_program(slots, steps, group, tile_bits=CHAIN_TILE_BITS):
     return words
 
 
+def chain_stages(words):
+    """(K bits, M bits) of every stage of a fused-chain program: 2^K x 2^M
+    is the operator shape the stage applies (e.g. (5, 3): 32 -> 8)."""
+    L, tb, nin, nout, nspec = words[:5]
+    q = 5 + 2 * nin + 2 * nout + 3 * nspec
+    out = []
+    for _ in range(L):
+        _, nk, nm, nr = words[q:q + 4]
+        out.append((nk, nm))
+        q += 4 + 2 * (nk + nm) + nr
+    return out
+
+
 def fuse_chains(slots, steps, dts, tile_bits=CHAIN_TILE_BITS, min_elems=CHAIN_MIN_ELEMS,
-                op_elems=CHAIN_OP_ELEMS):
+                op_elems=CHAIN_OP_ELEMS, group_bits=None):
     """Replace runs of complex64 steps that each contract a small operand
     (<= op_elems) with the previous step's large (>= min_elems) result by
     fused-chain steps (kind 2), as many consecutive steps per fused step as
-    fit a tile of 2^tile_bits.  Only unbatched steps that keep no sliced bond
-    take part.  Returns (steps, programs, nfused): programs[i] are the words
-    of step i (None for other steps), nfused[i] the contraction steps it
-    stands for."""
+    fit a tile of 2^tile_bits.  Only steps that run once per slice group
+    take part: their tensors depend on no sliced bond below group_bits
+    (None: on none at all) and keep none as an index -- except operators,
+    which may keep bonds above the group (read at each group's offset).
+    Returns (steps, programs, members): programs[i] are the words of step i
+    (None for other steps), members[i] the indices (into the input
+    `steps`) of the contraction steps it stands for ([] for a
+    conversion)."""
     reads = defaultdict(int)
     for kind, a, b, c, *_ in steps:
         reads[a] += 1
         if kind == 0 and b != a:
             reads[b] += 1
 
-    def plain(s):
+    gb = len(slots[0].slice_stride) if group_bits is None else group_bits
+
+    def plain(s, operator):
         sl = slots[s]
-        return not sl.deps and not sl.carry and not any(sl.slice_stride)
+        if any(j < gb for j in sl.deps):
+            return False
+        if operator:
+            return not any(v for j, v in enumerate(sl.slice_stride) if j < gb)
+        return not sl.carry and not any(sl.slice_stride)
 
     cand = {}
     for i, (kind, a, b, c, M, N, K) in enumerate(steps):
@@ -429,7 +463,7 @@ def fuse_chains(slots, steps, dts, tile_bits=CHAIN_TILE_BITS, min_elems=CHAIN_MI
         big, op = (a, b) if slots[a].elems >= slots[b].elems else (b, a)
         if slots[op].elems > op_elems or slots[big].elems < min_elems or slots[c].elems < min_elems:
             continue
-        if not (plain(a) and plain(b) and plain(c)):
+        if not (plain(big, False) and plain(op, True) and plain(c, False)):
             continue
         cand[i] = big
     # chains: a candidate continues the chain whose last result it reads
@@ -460,12 +494,12 @@ def fuse_chains(slots, steps, dts, tile_bits=CHAIN_TILE_BITS, min_elems=CHAIN_MI
             groups.append((ch[i:best[0] + 1], best[1]))
             i = best[0] + 1
     if not groups:
-        return steps, [None] * len(steps), [1 if s[0] == 0 else 0 for s in steps]
+        return steps, [None] * len(steps), [[i] if s[0] == 0 else [] for i, s in enumerate(steps)]
     drop, at = set(), {}
     for members, words in groups:
         drop |= set(members[:-1])
         at[members[-1]] = (members, words)
-    out, progs, nf = [], [], []
+    out, progs, mem = [], [], []
     for i, st in enumerate(steps):
         if i in drop:
             continue
@@ -476,12 +510,12 @@ def fuse_chains(slots, steps, dts, tile_bits=CHAIN_TILE_BITS, min_elems=CHAIN_MI
             op0 = first[2] if x0 == first[1] else first[1]
             out.append((2, x0, op0, st[3], [], [], []))
             progs.append(words)
-            nf.append(len(members))
+            mem.append(list(members))
         else:
             out.append(st)
             progs.append(None)
-            nf.append(1 if st[0] == 0 else 0)
-    return out, progs, nf
+            mem.append([i] if st[0] == 0 else [])
+    return out, progs, mem
 
 
 class Plan:
@@ -496,18 +530,45 @@ class Plan:
         self.n_sliced = n_sliced
         self.group_bits = group_bits
         self.dtypes = dtypes
+        self.chain_programs = [None] * len(steps)   # fused-chain program words per step
         out = slots[out_slot]
         self.out_shape = tuple(out.ext[x] for x in out.idx)
+        self.network = None
+        self.batch_handles = {}    # closed output leaf -> its bitstring-batch tensor
+        self.n_out_bits = 0
+        self.n_bitstrings = None
 
     def execute(self, use_graph=True):
         self.ctx.check(self.ctx.lib.pq_plan_execute(self.ctx.ptr, self.ptr, 1 if use_graph else 0))
 
     def result(self):
+        """The FP64 result in the output qubits' order; with a bitstring
+        batch, shape (number of bitstrings,) + that shape."""
         n = int(np.prod(self.out_shape)) if self.out_shape else 1
-        buf = np.empty(2 * n, dtype=np.float64)
+        nb = 1 << self.n_out_bits
+        buf = np.empty(2 * n * nb, dtype=np.float64)
         self.ctx.check(self.ctx.lib.pq_plan_result(self.ctx.ptr, self.ptr,
                                                    buf.ctypes.data_as(ct.POINTER(ct.c_double))))
-        return (buf[0::2] + 1j * buf[1::2]).reshape(self.out_shape, order="F")
+        z = buf[0::2] + 1j * buf[1::2]
+        if self.n_bitstrings is None:
+            return z.reshape(self.out_shape, order="F")
+        return np.stack([z[b * n:(b + 1) * n].reshape(self.out_shape, order="F")
+                         for b in range(self.n_bitstrings)])
+
+    def set_bitstrings(self, bitstrings):
+        """Rewrite the output batch in place (pq_tensor_write): the next runs,
+        graph replays included, compute these bitstrings.  Same count as
+        the plan was compiled with."""
+        if self.n_bitstrings is None:
+            raise ValueError("the plan was compiled without a bitstring batch")
+        bitstrings = list(bitstrings)
+        if len(bitstrings) != self.n_bitstrings:
+            raise ValueError(f"the plan computes {self.n_bitstrings} bitstrings per run")
+        data, _ = output_batch(self.network, bitstrings)
+        if set(data) != set(self.batch_handles):
+            raise ValueError("the bitstrings close other outputs than the plan's")
+        for lab, col in data.items():
+            self.ctx.write(self.batch_handles[lab], col)
 
     def stats(self):
         a, l, o = ct.c_int64(), ct.c_int64(), ct.c_int64()
@@ -533,12 +594,37 @@ class Plan:
         if self.ptr:
             self.ctx.lib.pq_plan_destroy(self.ctx.ptr, self.ptr)
             self.ptr = None
+        for h in self.batch_handles.values():
+            self.ctx.free(h)
+        self.batch_handles = {}
 
     def __del__(self):
         try:
             self.close()
         except Exception:
             pass
+
+
+def step_choices(pre_steps, members, kc=None, tiles=None):
+    """Kernel class and GEMM tile of every step after fusion.  kc / tiles
+    are per contraction step of the plan, i.e. per kind-0 step of
+    pre_steps (the step list before fusion) in order; members[i] names the
+    pre_steps a fused step stands for (fuse_chains), so a step between two
+    members of a chain keeps its own choice.  Chains and conversions: -1."""
+    plan_step, ci = {}, 0
+    for i, st in enumerate(pre_steps):
+        if st[0] == 0:
+            plan_step[i] = ci
+            ci += 1
+    kcs, tls = [], []
+    for mem in members:
+        if len(mem) == 1:
+            kcs.append(kc[plan_step[mem[0]]] if kc is not None else -1)
+            tls.append(tiles[plan_step[mem[0]]] if tiles is not None else -1)
+        else:
+            kcs.append(-1)
+            tls.append(-1)
+    return kcs, tls
 
 
 def _leaf_layout(backend):
@@ -568,9 +654,41 @@ def choose_group_bits(slots, n_sliced, budget_bytes, esize):
     return min(gb, 16)
 
 
+def output_batch(network, bitstrings):
+    """(leaf data, n_bits) of a batch of output bitstrings: for every closed
+    output qubit (a node made by add_output), a [2, 2^n_bits] array whose
+    column b is the qubit's output vector for bitstring b ('0' -> [1, 0],
+    '1' -> [0, 1], src/layer3.jl:302); the batch is padded to a power of two
+    with copies of the first bitstring.  bitstrings: strings over all qubits
+    in add_output's order ('x' or any character at an open qubit)."""
+    bitstrings = list(bitstrings)
+    if not bitstrings:
+        raise ValueError("empty bitstring batch")
+    n = network.number_qubits
+    if any(len(b) != n for b in bitstrings):
+        raise ValueError(f"every bitstring needs {n} characters")
+    n_bits = max(0, math.ceil(math.log2(len(bitstrings))))
+    padded = bitstrings + [bitstrings[0]] * ((1 << n_bits) - len(bitstrings))
+    data = {}
+    for i in range(n):
+        pos = network.qubit_ordering[i]
+        node = network.edges[network.output_qubits[pos - 1]].dst
+        if node is None or network.node_layers.get(node) != -1:
+            continue                       # open output
+        col = np.zeros((2, len(padded)), dtype=np.complex128)
+        for b, bits in enumerate(padded):
+            if bits[i] not in "01":
+                raise ValueError(f"bitstring {b} has {bits[i]!r} at closed qubit {i + 1}")
+            col[int(bits[i]), b] = 1.0
+        data[node] = col
+    if not data:
+        raise ValueError("the network has no closed outputs to batch over")
+    return data, n_bits
+
+
 def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=None, tiles=None,
                  budget_bytes=64 << 30, precision="uniform", fp64_steps=(), carry_elems=SMALL_ELEMS,
-                 small_elems=MIXED_FP64_ELEMS, fuse=True):
+                 small_elems=MIXED_FP64_ELEMS, fuse=True, bitstrings=None):
     """Compile a pair plan of a network whose backend is a HIPBackend.
 
     precision: "uniform" (every step in the backend's type), "mixed" (FP64
@@ -581,25 +699,44 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
     cores where their shape allows).  kc / tiles, when given, are per contraction step of
     `plan` (kernel class and GEMM tile, -1 = automatic).  fuse: runs of
     complex64 steps that apply small operators to one large tensor become
-    fused-chain steps (fuse_chains; a dict passes its limits)."""
+    fused-chain steps (fuse_chains; a dict passes its limits).
+    bitstrings: a batch of output bitstrings (output_batch) computed in one
+    run, one result per bitstring; the closed output qubits' leaves are
+    replaced by device tensors with a column per bitstring, which
+    Plan.set_bitstrings rewrites in place for the next runs."""
     backend = network.backend
     ctx = backend.ctx
     handles, strides = _leaf_layout(backend)
     missing = [lab for lab in network.nodes if lab not in handles]
     if missing:
         raise KeyError(f"leaf tensors missing on the device: {missing[:4]}")
-    slots, steps, out_slot = lower_plan(network, plan, sliced, out_order, handles, strides, carry_elems)
-    n_sliced = len(list(sliced))
+    extra, batch_handles, n_out_bits, n_bits_in = {}, {}, 0, None
+    if bitstrings is not None:
+        data, n_out_bits = output_batch(network, bitstrings)
+        n_bits_in = len(list(bitstrings))
+        for lab, col in data.items():
+            h = ctx.upload(col)
+            batch_handles[lab] = h
+            handles[lab], strides[lab] = h, [1]
+            extra[lab] = [2 << k for k in range(n_out_bits)]
+    slots, steps, out_slot = lower_plan(network, plan, sliced, out_order, handles, strides, carry_elems,
+                                        extra_bonds=extra)
+    n_real = len(list(sliced))
+    n_sliced = n_real + n_out_bits
     split16 = precision == "mixed16"
     dts, steps, slots = assign_precision(slots, steps, ctx.code, "mixed" if split16 else precision,
                                          small_elems=small_elems, fp64_steps=fp64_steps)
     if group_bits is None:
         esize = 8 if precision in ("mixed", "mixed16") or ctx.code == _lib.PQ_C64 else 16
-        group_bits = choose_group_bits(slots, n_sliced, budget_bytes, esize)
+        group_bits = choose_group_bits(slots, n_real, budget_bytes, esize)
+    if group_bits > n_real:
+        raise ValueError("the slice group cannot include output-batch bonds")
+    pre_steps = steps
     if fuse:
-        steps, progs, nfused = fuse_chains(slots, steps, dts, **(fuse if isinstance(fuse, dict) else {}))
+        steps, progs, members = fuse_chains(slots, steps, dts, group_bits=group_bits,
+                                            **(fuse if isinstance(fuse, dict) else {}))
     else:
-        progs, nfused = [None] * len(steps), [1 if st[0] == 0 else 0 for st in steps]
+        progs, members = [None] * len(steps), [[i] if st[0] == 0 else [] for i, st in enumerate(steps)]
     slot_elems = [s.elems for s in slots]
     slot_mask = [0 if s.leaf else sum(1 << j for j in s.deps) for s in slots]
     slot_leaf = [s.leaf for s in slots]
@@ -607,12 +744,9 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
     for s in slots:
         lss += (list(s.slice_stride) if s.slice_stride else [0] * n_sliced)
     step_slots, nmodes, modes, kinds, ariths = [], [], [], [], []
-    kcs_in = list(kc) if kc is not None else None
-    tls_in = list(tiles) if tiles is not None else None
-    kcs, tls = [], []
-    ci = 0
+    kcs, tls = step_choices(pre_steps, members, kc, tiles)
     chain_off, words = [], []
-    for (kind, sa, sb, sc, M, N, K), prog, nf in zip(steps, progs, nfused):
+    for (kind, sa, sb, sc, M, N, K), prog in zip(steps, progs):
         step_slots += [sa, sb, sc]
         nmodes += [len(M), len(N), len(K)]
         kinds.append(kind)
@@ -623,13 +757,6 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
         chain_off.append(len(words) if prog is not None else -1)
         if prog is not None:
             words += prog
-        if kind == 0:
-            kcs.append(kcs_in[ci] if kcs_in else -1)
-            tls.append(tls_in[ci] if tls_in else -1)
-        else:
-            kcs.append(-1)
-            tls.append(-1)
-        ci += nf
     keep = dict(
         slot_elems=_arr(ct.c_int64, slot_elems), slot_mask=_arr(ct.c_int64, slot_mask),
         slot_leaf=_arr(ct.c_int64, slot_leaf), lss=_arr(ct.c_int64, lss or [0]),
@@ -640,7 +767,18 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
     d = PlanDesc(len(slots), keep["slot_elems"], keep["slot_mask"], keep["slot_leaf"], n_sliced, keep["lss"],
                  group_bits, len(steps), keep["step_slots"], keep["kc"], keep["tile"], keep["nmodes"],
                  keep["modes"], out_slot, keep["dt"], keep["kind"], keep["arith"], keep["chain_off"],
-                 len(words), keep["words"])
+                 len(words), keep["words"], n_out_bits)
     p = ct.c_void_p()
-    ctx.check(ctx.lib.pq_plan_create(ctx.ptr, ct.cast(ct.pointer(d), ct.c_void_p), ct.byref(p)))
-    return Plan(ctx, p, slots, steps, out_slot, n_sliced, group_bits, dts)
+    try:
+        ctx.check(ctx.lib.pq_plan_create(ctx.ptr, ct.cast(ct.pointer(d), ct.c_void_p), ct.byref(p)))
+    except Exception:
+        for h in batch_handles.values():
+            ctx.free(h)
+        raise
+    out = Plan(ctx, p, slots, steps, out_slot, n_sliced, group_bits, dts)
+    out.chain_programs = progs
+    out.network = network
+    out.batch_handles = batch_handles
+    out.n_out_bits = n_out_bits
+    out.n_bitstrings = n_bits_in
+    return out

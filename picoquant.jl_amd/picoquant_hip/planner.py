@@ -3,10 +3,12 @@
 
   bgreedy        -- Boltzmann-greedy with the Gray-Kourtis cost |C| - alpha(|A|+|B|)
                     (src/layer2/bgreedy_contraction.jl:17-131).  tau = 0 is the
-                    deterministic limit (argmin, first pair in scan order on ties);
-                    tau > 0 samples through the Gumbel-max form of the same
-                    Boltzmann weights, which cannot overflow (the reference's
-                    exp(-E/tau) does at 2^30-element tensors).
+                    deterministic limit (argmin over all pairs, first pair in
+                    scan order on ties, where the reference's limit would draw
+                    uniformly among the tied pairs); tau > 0 samples through the
+                    Gumbel-max form of the same Boltzmann weights, which cannot
+                    overflow (the reference's exp(-E/tau) does at 2^30-element
+                    tensors).
   sweep_plan     -- world lines first (each qubit's tensors contracted along
                     time into one site tensor), then the sites absorbed one by
                     one into a growing boundary in a given site order.
@@ -87,12 +89,39 @@ def _bgreedy_scan(tensors, counter, alpha, tau, rng):
     return plan, tcost, scost
 
 
+def _outer_energy_bound(sizes, alpha):
+    """A lower bound on |A||B| - alpha(|A| + |B|) over all pairs of distinct
+    tensors whose sizes are the multiset `sizes` (Counter size -> count):
+    the energy of any pair that shares no index.  g(a, b) = ab - alpha(a + b)
+    is linear in each argument, so its minimum over pairs lies at a pair of
+    extremes: the two smallest, the smallest and the largest, or the two
+    largest sizes."""
+    ks = sorted(sizes)
+    s1, sn = ks[0], ks[-1]
+    s2 = s1 if sizes[s1] > 1 else ks[1]
+    sm = sn if sizes[sn] > 1 else ks[-2]
+
+    def g(a, b):
+        return a * b - alpha * (a + b)
+
+    return min(g(s1, s2), g(s1, sn), g(sm, sn))
+
+
 def _bgreedy_heap(tensors, counter, alpha, tau, rng):
-    """Same selection restricted to pairs that share an index (disconnected
-    pieces are joined by outer products at the end)."""
+    """The selection of _bgreedy_scan without the O(N^2) scan per step.
+    Pairs that share an index wait in a heap keyed by (energy, position of
+    A, position of B) -- positions follow creation order, so ties break as
+    the scan's first pair does.  tau = 0: the heap's top is the all-pairs
+    argmin whenever its energy is below _outer_energy_bound, which bounds
+    every pair that shares nothing; otherwise (on grid circuits: at most
+    once per plan) that step scans all pairs.  tau > 0: samples among the
+    pairs that share an index (Gumbel-max over the heap), not among all
+    pairs."""
     order = {k: i for i, k in enumerate(tensors)}
     ix2t = defaultdict(set)
-    for k, (idx, _) in tensors.items():
+    sizes = defaultdict(int)
+    for k, (idx, dims) in tensors.items():
+        sizes[math.prod(dims)] += 1
         for x in idx:
             ix2t[x].add(k)
     heap = []
@@ -110,10 +139,12 @@ def _bgreedy_heap(tensors, counter, alpha, tau, rng):
     plan, tcost, scost = [], 0.0, 0
     nxt = len(order)
     while len(tensors) > 1:
-        if heap:
+        while heap and (heap[0][3] not in tensors or heap[0][4] not in tensors):
+            heapq.heappop(heap)
+        if heap and (tau > 0 or heap[0][0] < _outer_energy_bound(sizes, alpha)):
             _, _, _, a, b = heapq.heappop(heap)
-            if a not in tensors or b not in tensors:
-                continue
+        elif tau <= 0:
+            a, b = _scan_argmin(tensors, alpha)
         else:
             it = iter(tensors)
             a, b = next(it), next(it)
@@ -121,6 +152,11 @@ def _bgreedy_heap(tensors, counter, alpha, tau, rng):
         sa, sb = math.prod(A[1]), math.prod(B[1])
         C = _merge(A, B)
         sc = math.prod(C[1])
+        for v in (sa, sb):
+            sizes[v] -= 1
+            if not sizes[v]:
+                del sizes[v]
+        sizes[sc] += 1
         counter += 1
         c = f"node_{counter}"
         for x in A[0]:
@@ -144,6 +180,19 @@ def _bgreedy_heap(tensors, counter, alpha, tau, rng):
         tcost += math.sqrt(sa * sb * sc)
         scost = max(scost, sc)
     return plan, tcost, scost
+
+
+def _scan_argmin(tensors, alpha):
+    """(A, B): the minimum-energy pair of all, first in scan order on ties."""
+    items = list(tensors.items())
+    best = None
+    for i in range(len(items)):
+        ai, ad = items[i][1]
+        for j in range(i + 1, len(items)):
+            e = _pair_cost(ai, ad, *items[j][1], alpha)[0]
+            if best is None or e < best[0]:
+                best = (e, items[i][0], items[j][0])
+    return best[1], best[2]
 
 
 def bgreedy(network, alpha=1.0, tau=1.0, N=10, rng=None):
@@ -741,7 +790,12 @@ def choose_slices(network, plan, n_bonds, max_elems=2 ** 28, exclude=(), candida
         over = peak > max_elems
         key = np.where(over, np.inf, cost)
         if np.all(np.isinf(key)):
-            key = cost
+            # no bond meets the cap yet: lower the largest per-slice tensor
+            # first (a bond that misses it cannot help later rounds either),
+            # the cheapest such bond on ties
+            pk = peak.astype(float)
+            pk[chosen] = np.inf
+            key = np.where(pk <= pk.min(), cost, np.inf)
         chosen.append(int(np.argmin(key)))
     smask = 0
     for q in chosen:

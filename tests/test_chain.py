@@ -106,10 +106,11 @@ def test_fused_chains_reproduce_the_unfused_steps(tile_bits):
     slots, steps, out = lower_plan(tn, plan)
     # complex64 above 16 elements, so most steps qualify
     dts, steps, slots = assign_precision(slots, steps, _lib.PQ_C128, "mixed", small_elems=16)
-    fused, progs, nf = fuse_chains(slots, steps, dts, tile_bits=tile_bits, min_elems=64, op_elems=256)
+    fused, progs, members = fuse_chains(slots, steps, dts, tile_bits=tile_bits, min_elems=64, op_elems=256)
     n_chain = sum(1 for p in progs if p is not None)
     assert n_chain >= 1 and len(fused) < len(steps)
-    assert sum(nf) == sum(1 for s in steps if s[0] == 0)
+    contract = [i for i, s in enumerate(steps) if s[0] == 0]
+    assert sorted(j for m in members for j in m) == contract
     for p in progs:
         if p is not None:
             assert p[1] <= tile_bits and 2 <= p[0] <= _lib.PQ_CHAIN_MAX_STAGES
@@ -138,3 +139,32 @@ def test_chain_program_limits():
     # the whole network as one chain exceeds any tile
     idx = [i for i, s in enumerate(steps) if s[0] == 0]
     assert chain_program(slots, steps, idx[-3:], tile_bits=2) is None
+
+
+def test_step_choices_follow_the_members_of_each_fused_step():
+    """Per-step kernel choices after fusion come from the plan step each
+    fused step stands for, also when an unrelated contraction step falls
+    between two members of one chain."""
+    from picoquant_hip.executor import step_choices
+    # before fusion: contraction steps c0 c1 c2 c3 (plan steps 0..3) and a
+    # conversion between c1 and c2; c0 and c2 form one chain
+    pre = [(0,), (0,), (1,), (0,), (0,)]
+    members = [[1], [], [0, 3], [4]]          # c1, the conversion, chain(c0, c2), c3
+    kcs, tls = step_choices(pre, members, kc=[10, 11, 12, 13], tiles=[20, 21, 22, 23])
+    assert kcs == [11, -1, -1, 13]
+    assert tls == [21, -1, -1, 23]
+    assert step_choices(pre, members) == ([-1] * 4, [-1] * 4)
+    # and on a real fused plan: every unfused contraction step keeps its own
+    cfg = "0110" + "x" * 4 + "1001" + "01" + "x" * 2
+    tn, _, _ = _grid(4, 4, 8, 7, cfg)
+    plan = PL.reconfigure(tn, PL.sweep_plan(tn, PL.grid_site_order(4, 4, "col")), windows=(8,))
+    slots, steps, out = lower_plan(tn, plan)
+    dts, steps, slots = assign_precision(slots, steps, _lib.PQ_C128, "mixed", small_elems=16)
+    fused, progs, members = fuse_chains(slots, steps, dts, tile_bits=10, min_elems=64, op_elems=256)
+    kcs, _ = step_choices(steps, members, kc=list(range(len(plan))))
+    contract = [i for i, s in enumerate(steps) if s[0] == 0]
+    for st, mem, k in zip(fused, members, kcs):
+        if st[0] == 0:
+            assert k == contract.index(mem[0])
+        else:
+            assert k == -1

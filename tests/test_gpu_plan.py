@@ -162,3 +162,32 @@ def test_grid5x5_d12_mixed_precision(golden_dir, nslice, group_bits):
         p.execute(True)
         assert rel(p.result(), want) < RTOL["mixed"]
     p.close()
+
+
+def test_plan_keeps_its_leaves_after_they_are_freed():
+    """A compiled plan holds its leaves' device buffers: deleting leaf
+    tensors (as contract_tensors does with its operands) or reshaping them,
+    then allocating over the freed memory, leaves every later run -- eager
+    and graph -- with the same amplitudes."""
+    cfg = "1x0x11x01"
+    tn = grid_net(3, 3, 6, 21, cfg, np.complex128)
+    open_q = [i + 1 for i, ch in enumerate(cfg) if ch == "x"]
+    plan = PL.sweep_plan(tn, PL.grid_site_order(3, 3, open_last=open_q))
+    sliced = PL.choose_slices(tn, plan, 2)[0]
+    p = compile_plan(tn, plan, sliced, group_bits=1)
+    p.execute(False)
+    want = p.result().copy()
+    be, ctx = tn.backend, tn.backend.ctx
+    labels = list(tn.nodes)
+    for lab in labels[: len(labels) // 2]:
+        be.delete_tensor(lab)
+    for lab in labels[len(labels) // 2:]:
+        if len(tn.nodes[lab].dims) > 1:
+            ctx.reshape(be.tensors[lab], [int(np.prod(tn.nodes[lab].dims))])
+    junk = [ctx.upload(np.full(64, 7 + 7j, dtype=np.complex128)) for _ in range(256)]
+    for use_graph in (False, True, True):
+        p.execute(use_graph)
+        np.testing.assert_array_equal(p.result(), want)
+    for h in junk:
+        ctx.free(h)
+    p.close()

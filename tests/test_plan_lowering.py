@@ -162,3 +162,54 @@ def test_slicing_candidates():
         assert len(set(bonds)) == 3 and peak <= 2 ** 10
         smask = sum(1 << ids[b] for b in bonds)
         assert (tot, peak) == PL.sliced_cost(steps, ext, smask, 8, "macs")
+
+
+@pytest.mark.parametrize("nslice,carry", [(0, None), (3, None), (3, 0)])
+def test_lowered_bitstring_batch(nslice, carry):
+    """A batch of output bitstrings lowered as extra bonds kept by the closed
+    output leaves (executor.output_batch / lower_plan extra_bonds): summing
+    each bitstring's slices reproduces its brute-force amplitudes."""
+    from picoquant_hip.executor import output_batch
+    rows, cols, depth, seed = 3, 3, 5, 4
+    cfg = "10x1x0110"
+    sv = O.statevector(9, O.grid_circuit_gates(rows, cols, depth, seed))
+    rng = np.random.default_rng(11)
+    batch = ["".join("x" if c == "x" else str(int(rng.integers(0, 2))) for c in cfg) for _ in range(3)]
+    tn, data, _ = build(rows, cols, depth, seed, batch[0])
+    cols_data, n_bits = output_batch(tn, batch)
+    assert n_bits == 2 and all(v.shape == (2, 4) for v in cols_data.values())
+    labels = list(tn.nodes)
+    extra = {lab: [2 << k for k in range(n_bits)] for lab in cols_data}
+    leaf_data = [cols_data[lab].reshape(-1, order="F") if lab in cols_data else data[i]
+                 for i, lab in enumerate(labels)]
+    strides = {lab: ([1] if lab in cols_data else [int(np.prod(tn.nodes[lab].dims[:i]))
+                                                    for i in range(len(tn.nodes[lab].dims))]) for lab in labels}
+    open_q = [i + 1 for i, ch in enumerate(cfg) if ch == "x"]
+    plan = PL.sweep_plan(tn, PL.grid_site_order(rows, cols, open_last=open_q))
+    sliced = PL.choose_slices(tn, plan, nslice)[0] if nslice else []
+    kw = {} if carry is None else {"carry_elems": carry}
+    slots, steps, out_slot = lower_plan(tn, plan, sliced, leaf_strides=strides, extra_bonds=extra, **kw)
+    nb = len(sliced) + n_bits
+    assert sorted(slots[out_slot].deps) == list(range(nb))
+    for b, bits_b in enumerate(batch):
+        total = 0
+        for s in range(2 ** len(sliced)):
+            v = [(s >> j) & 1 for j in range(len(sliced))] + [(b >> k) & 1 for k in range(n_bits)]
+            mem = {i: leaf_data[i] for i in range(len(labels))}
+            for sa, sb, sc, M, N, K in steps:
+                def off(x):
+                    if not slots[sc].dep:
+                        return 0
+                    return sum(bit * st for bit, st in zip(v, slots[x].slice_stride))
+                Am = mem[sa][off(sa) + offsets(M, 0)[:, None] + offsets(K, 0)[None, :]]
+                Bm = mem[sb][off(sb) + offsets(K, 1)[:, None] + offsets(N, 0)[None, :]]
+                out = np.zeros(slots[sc].elems, dtype=np.complex128)
+                out[offsets(M, 1)[:, None] + offsets(N, 1)[None, :]] = Am @ Bm
+                mem[sc] = out
+            total = total + mem[out_slot]
+        for j in range(4):
+            full = list(bits_b)
+            full[open_q[0] - 1] = str(j & 1)
+            full[open_q[1] - 1] = str((j >> 1) & 1)
+            want = sv[sum(int(c) << q for q, c in enumerate(full))]
+            assert abs(total[j] - want) < 1e-13
