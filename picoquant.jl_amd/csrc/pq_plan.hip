@@ -36,38 +36,55 @@ __global__ void k_fill_zero(double* p, int64_t n) {
 }
 
 // out[j] += sum_b C[b * stride + j] for b < nb, in FP64 and in a fixed
-// order (the same bits on every run).  A workgroup of 16 x 16 lanes takes 16
-// consecutive outputs: lane (tb, tj) sums entries b = tb, tb + 16, ... of
-// output j0 + tj (16 lanes of one b read 16 adjacent elements), then the 16
-// partial sums of each output are added pairwise in LDS.
+// order (the same bits on every run).  A workgroup of 256 lanes takes tj
+// consecutive outputs with tb = 256 / tj lanes each (tj a power of two,
+// chosen by the host: few outputs with many copies give one output 256
+// lanes, many outputs give 256 outputs one lane each): lane (b0, j) sums
+// entries b = b0, b0 + tb, ... of its output (the tj lanes of one b0 read
+// adjacent elements), then the tb partial sums of each output are added
+// pairwise in LDS.
 template <typename C>
 __global__ __launch_bounds__(256) void k_accum(const C* __restrict__ src, int64_t elems, int64_t stride, int nb,
-                                               double* __restrict__ out) {
-  __shared__ double pr[16][17], pi[16][17];
-  const int tj = threadIdx.x % 16, tb = threadIdx.x / 16;
-  const int64_t j = (int64_t)blockIdx.x * 16 + tj;
+                                               int log_tj, double* __restrict__ out) {
+  __shared__ double pr[256], pi[256];
+  const int tj = 1 << log_tj, tb = 256 >> log_tj;
+  const int lj = threadIdx.x & (tj - 1), b0 = threadIdx.x >> log_tj;
+  const int64_t j = (int64_t)blockIdx.x * tj + lj;
   double re = 0, im = 0;
   if (j < elems) {
-    for (int b = tb; b < nb; b += 16) {
+    for (int b = b0; b < nb; b += tb) {
       C v = src[b * stride + j];
       re += (double)v.x;
       im += (double)v.y;
     }
   }
-  pr[tb][tj] = re;
-  pi[tb][tj] = im;
+  pr[threadIdx.x] = re;
+  pi[threadIdx.x] = im;
   __syncthreads();
-  for (int h = 8; h > 0; h >>= 1) {
-    if (tb < h) {
-      pr[tb][tj] += pr[tb + h][tj];
-      pi[tb][tj] += pi[tb + h][tj];
+  for (int h = 128; h >= tj; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      pr[threadIdx.x] += pr[threadIdx.x + h];
+      pi[threadIdx.x] += pi[threadIdx.x + h];
     }
     __syncthreads();
   }
-  if (tb == 0 && j < elems) {
-    out[2 * j] += pr[0][tj];
-    out[2 * j + 1] += pi[0][tj];
+  if (b0 == 0 && j < elems) {
+    out[2 * j] += pr[lj];
+    out[2 * j + 1] += pi[lj];
   }
+}
+
+// log2 of the outputs per k_accum workgroup (tj = 256 / tb): tb, the lanes
+// per output, is about nb / 4 (up to 256), so each lane adds about four
+// entries; with 1024 outputs or more at most 16, so that a workgroup's
+// loads cover 16 or more adjacent outputs.
+int accum_log_tj(int64_t elems, int nb) {
+  int tb = 1;
+  while (tb < 256 && (int64_t)tb * 4 < nb) tb <<= 1;
+  if (elems >= 1024 && tb > 16) tb = 16;
+  int lt = 0;
+  while ((256 >> lt) > tb) ++lt;
+  return lt;
 }
 
 // Element type conversion, n elements, same layout.
@@ -574,13 +591,15 @@ int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
     const void* src = slot_ptr(c, p, p->out_slot, grp);
     // the group's output batch: its bits above the summed slice bonds
     double* dst = p->out_dev + 2 * p->out_elems * (grp >> (p->n_sliced - p->n_out_bits - p->group_bits));
-    const unsigned ga = (unsigned)((os.elems + 15) / 16);
+    const int lt = accum_log_tj(os.elems, nbatch);
+    const unsigned ga = (unsigned)((os.elems + (1 << lt) - 1) >> lt);
+    ProfScope ps(c, PQ_KC_ACCUM, (double)os.elems * nbatch * esize_of(os.dtype) + 16.0 * os.elems, 0);
     if (os.dtype == PQ_C128)
       hipLaunchKernelGGL((k_accum<c128>), dim3(ga), dim3(256), 0, c->stream, (const c128*)src,
-                         os.elems, os.elems, nbatch, dst);
+                         os.elems, os.elems, nbatch, lt, dst);
     else
       hipLaunchKernelGGL((k_accum<c64>), dim3(ga), dim3(256), 0, c->stream, (const c64*)src,
-                         os.elems, os.elems, nbatch, dst);
+                         os.elems, os.elems, nbatch, lt, dst);
     if (hipGetLastError() != hipSuccess) return fail(c, PQ_ERR_HIP, "accumulate kernel");
   }
   return PQ_OK;

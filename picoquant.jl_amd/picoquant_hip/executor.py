@@ -97,9 +97,13 @@ def _assign_strides(C):
 
 
 def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf_strides=None,
-               carry_elems=SMALL_ELEMS, extra_bonds=None):
+               carry_elems=SMALL_ELEMS, extra_bonds=None, leaf_dims=None):
     """Pure-host lowering (no device calls): returns (slots, steps, out_slot)
     where steps = [(a, b, c, M, N, K)] with modes as (ext, s0, s1).
+
+    leaf_dims: {leaf label: dims of its stored tensor}, when they differ
+    from the Node's (a view made by slice_tensor_network has extent 1 on
+    the sliced bond, while its Node keeps the old dims, as in the reference).
 
     extra_bonds: {leaf label: [stride of bond k, k = 0..n-1]} -- n more
     extent-2 bonds after the sliced ones that are no edge of the network
@@ -121,7 +125,7 @@ def lower_plan(network, plan, sliced=(), out_order=None, leaf_handles=None, leaf
     label_slot = {}
     for lab, node in network.nodes.items():
         s = SlotInfo()
-        dims = list(node.dims)
+        dims = list(leaf_dims[lab]) if leaf_dims is not None and lab in leaf_dims else list(node.dims)
         if leaf_strides is not None:
             strides = list(leaf_strides[lab])
         else:
@@ -628,17 +632,19 @@ def step_choices(pre_steps, members, kc=None, tiles=None):
 
 
 def _leaf_layout(backend):
-    handles, strides = {}, {}
+    """Handles, strides and dims of the backend's device tensors."""
+    handles, strides, dims = {}, {}, {}
     ctx = backend.ctx
     for lab, h in backend.tensors.items():
         base = ct.c_void_p()
         off = ct.c_int64()
         st = (ct.c_int64 * _lib.PQ_MAX_RANK)()
         ctx.check(ctx.lib.pq_tensor_device_ptr(ctx.ptr, h, ct.byref(base), ct.byref(off), st))
-        n = len(ctx.shape(h))
+        shp = ctx.shape(h)
         handles[lab] = h
-        strides[lab] = [int(st[i]) for i in range(n)]
-    return handles, strides
+        strides[lab] = [int(st[i]) for i in range(len(shp))]
+        dims[lab] = list(shp)
+    return handles, strides, dims
 
 
 def choose_group_bits(slots, n_sliced, budget_bytes, esize):
@@ -706,7 +712,7 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
     Plan.set_bitstrings rewrites in place for the next runs."""
     backend = network.backend
     ctx = backend.ctx
-    handles, strides = _leaf_layout(backend)
+    handles, strides, dims = _leaf_layout(backend)
     missing = [lab for lab in network.nodes if lab not in handles]
     if missing:
         raise KeyError(f"leaf tensors missing on the device: {missing[:4]}")
@@ -717,10 +723,13 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
         for lab, col in data.items():
             h = ctx.upload(col)
             batch_handles[lab] = h
-            handles[lab], strides[lab] = h, [1]
+            handles[lab], strides[lab], dims[lab] = h, [1], [2]
             extra[lab] = [2 << k for k in range(n_out_bits)]
+    for lab, node in network.nodes.items():
+        if len(dims[lab]) != len(node.dims):
+            raise ValueError(f"leaf {lab}: the stored tensor has rank {len(dims[lab])}, the node {len(node.dims)}")
     slots, steps, out_slot = lower_plan(network, plan, sliced, out_order, handles, strides, carry_elems,
-                                        extra_bonds=extra)
+                                        extra_bonds=extra, leaf_dims=dims)
     n_real = len(list(sliced))
     n_sliced = n_real + n_out_bits
     split16 = precision == "mixed16"

@@ -191,3 +191,31 @@ def test_plan_keeps_its_leaves_after_they_are_freed():
     for h in junk:
         ctx.free(h)
     p.close()
+
+
+def test_plan_on_a_network_sliced_by_the_reference_driver():
+    """compile_plan on a network after slice_tensor_network: the views have
+    extent 1 on the sliced bond while their Nodes keep the old dims (as in
+    the reference, src/layer2/slicing.jl:67-93); the plan lowers the stored
+    shapes and equals contract_network! on the same slice."""
+    cfg = "0x1x01"
+    for s in range(4):
+        vals = [(s & 1) + 1, ((s >> 1) & 1) + 1]
+        t1 = grid_net(2, 3, 5, 6, cfg, np.complex128)
+        open_q = [i + 1 for i, ch in enumerate(cfg) if ch == "x"]
+        plan = PL.sweep_plan(t1, PL.grid_site_order(2, 3, open_last=open_q))
+        sliced = PL.choose_slices(t1, plan, 2)[0]
+        from picoquant_hip.slicing import slice_with_map, translate_plan
+        old = t1.counters["node"]
+        m = slice_with_map(t1, sliced, vals)
+        plan1 = translate_plan(plan, m, old, t1.counters["node"])
+        p = compile_plan(t1, plan1)
+        p.execute(True)
+        got = p.result().reshape(-1, order="F")
+        p.close()
+        t2 = grid_net(2, 3, 5, 6, cfg, np.complex128)
+        P.slice_tensor_network(t2, sliced, vals)
+        P.contract_network(t2, plan1, "vector")
+        want = np.asarray(t2.backend.load_tensor_data("result")).reshape(-1, order="F")
+        # a slice can close off amplitudes exactly: compare against the largest
+        assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
