@@ -16,9 +16,13 @@ import math
 import numpy as np
 
 from . import _lib
-from .executor import SMALL_ELEMS, assign_precision, lower_plan
+from .executor import SMALL_ELEMS, assign_precision, fuse_chains, lower_plan
 
 N_FEATURES = 16
+# Fused chains (PQ_KC_CHAIN) are priced at this rate of complex MACs: 8 real
+# flops each at 0.44 of the FP32 MFMA peak, the kernel's measured rate on
+# the 36-qubit depth-14 plan (0.42-0.45); the step model never sees them.
+CHAIN_MACS_PER_US = 0.44 * 157.3e12 / 8 / 1e6
 
 
 def merge(modes):
@@ -57,16 +61,26 @@ def step_features(M, N, K, batch=1, dtype=_lib.PQ_C64):
     ]
 
 
-def plan_samples(network, plan, sliced=(), group_bits=None, precision="mixed", max_elems=1 << 27):
+def plan_samples(network, plan, sliced=(), group_bits=None, precision="mixed", max_elems=1 << 27, fused=None):
     """(features, (M, N, K, batch, dtype)) for every contraction step of a
-    lowered plan, with slice-dependent steps batched over 2^group_bits."""
+    lowered plan, with slice-dependent steps batched over 2^group_bits.
+    fused: a list that receives the MAC count of every fused chain the
+    executor would form (fuse_chains); those chains' steps are left out."""
     slots, steps, out_slot = lower_plan(network, plan, sliced)
     dts, steps, slots = assign_precision(slots, steps, _lib.PQ_C128, precision)
     nsl = len(list(sliced))
     gb = nsl if group_bits is None else group_bits
+    skip = set()
+    if fused is not None:
+        _, progs, members = fuse_chains(slots, steps, dts, group_bits=gb)
+        for prog, mem in zip(progs, members):
+            if prog is None:
+                continue
+            skip |= set(mem)
+            fused.append(sum(math.prod(m[0] for m in steps[j][4] + steps[j][5] + steps[j][6]) for j in mem))
     out = []
-    for kind, sa, sb, sc, M, N, K in steps:
-        if kind != 0:
+    for i, (kind, sa, sb, sc, M, N, K) in enumerate(steps):
+        if kind != 0 or i in skip:
             continue
         if max(slots[x].elems for x in (sa, sb, sc)) > max_elems:
             continue
@@ -152,16 +166,19 @@ class MLPCostModel:
         return np.exp(self.predict_log_us(X).astype(np.float64))
 
     def plan_time(self, network, plan, sliced=(), precision="mixed"):
-        """Predicted microseconds of one planned run: individually launched
+        """Predicted microseconds of one planned run as compile_plan would
+        run it: runs of steps the executor fuses into chains at the chain
+        kernel's rate (CHAIN_MACS_PER_US), the other individually launched
         steps at their predicted times, slice-dependent steps once per slice
         group; small steps share grouped launches (one per dependency level),
         charged at their largest predicted step."""
-        samples = plan_samples(network, plan, sliced, None, precision, max_elems=1 << 34)
+        chains = []
+        samples = plan_samples(network, plan, sliced, None, precision, max_elems=1 << 34, fused=chains)
+        total = sum(chains) / CHAIN_MACS_PER_US
         if not samples:
-            return 0.0
+            return total
         X = np.array([f for f, _ in samples], dtype=np.float32)
         t = self.predict_us(X)
-        total = 0.0
         small = []
         for (f, (M, N, K, batch, dt)), ti in zip(samples, t):
             mt = 2 ** (f[0] + f[1])

@@ -66,9 +66,13 @@ def candidate_plans(tn, rows, cols, open_q, n_candidates=64, seed=0, n_greedy=2,
 
 
 def select_plan(tn, rows, cols, open_q, n_sliced, model=None, n_candidates=64, shortlist=8,
-                max_elems=2 ** 28, seed=0):
+                max_elems=2 ** 28, seed=0, measure=None, n_measure=3):
     """Pick a plan + slicing.  model: a trained costmodel.MLPCostModel, or
-    None to rank by the roofline-time slicing objective alone."""
+    None to rank by the roofline-time slicing objective alone.  measure:
+    callable (plan, sliced) -> measured ms; when given, the n_measure best
+    predicted candidates and the FLOP pick are run and the fastest measured
+    one is chosen (the model prunes 64 candidates to a few runs; measured
+    time has the last word)."""
     cands = candidate_plans(tn, rows, cols, open_q, n_candidates, seed)
     rows_out = []
     for name, plan in cands:
@@ -92,15 +96,26 @@ def select_plan(tn, rows, cols, open_q, n_sliced, model=None, n_candidates=64, s
         r["sliced"], r["sliced_macs"], r["peak"] = sl, tot, peak
         r["pred_sliced_us"] = model.plan_time(tn, r["plan"], sl) if model is not None else tot
     ok = [r for r in short if r.get("sliced") is not None and r["peak"] <= max_elems]
+    measured = {}
+    model_pick = None
     if ok:
         # predictions within 2% of the best count as a tie, broken by the
         # MAC count and then by candidate order (a model retrained on fresh
         # timings then keeps picking the same plan)
         top = min(r["pred_sliced_us"] for r in ok)
         tied = [r for r in ok if r["pred_sliced_us"] <= top * 1.02]
-        best = min(tied, key=lambda r: r["sliced_macs"])
+        best = model_pick = min(tied, key=lambda r: r["sliced_macs"])
         flop_pick = min(ok, key=lambda r: r["sliced_macs"])
         fallback = False
+        if measure is not None:
+            runs = [best] + [r for r in sorted(ok, key=lambda r: r["pred_sliced_us"]) if r is not best]
+            runs = runs[:n_measure]
+            if flop_pick not in runs:
+                runs.append(flop_pick)
+            for r in runs:
+                r["measured_ms"] = float(measure(r["plan"], r["sliced"]))
+                measured[r["name"]] = round(r["measured_ms"], 4)
+            best = min(runs, key=lambda r: r["measured_ms"])
     else:
         sl, tot, peak = PL.choose_slices(tn, greedy["plan"], n_sliced, max_elems=max_elems)
         greedy.update(sliced=sl, sliced_macs=tot, peak=peak, pred_sliced_us=math.nan)
@@ -114,10 +129,15 @@ def select_plan(tn, rows, cols, open_q, n_sliced, model=None, n_candidates=64, s
         "peak_slice_elems_log2": round(math.log2(best["peak"]), 2),
         "predicted_ms": None if model is None else round(best["pred_sliced_us"] / 1e3, 3),
         "flop_pick": flop_pick["name"],
+        "model_pick": model_pick["name"] if model_pick is not None else None,
+        "measured_ms": measured,
         "greedy_unsliced_macs_log2": round(math.log2(greedy["macs"]), 2),
         "shortlist": [(r["name"], round(math.log2(r["macs"]), 2),
                        None if model is None else round(r.get("pred_sliced_us", math.inf) / 1e3, 3))
                       for r in short],
     }
     return {"plan": best["plan"], "sliced": best["sliced"], "summary": summary,
-            "flop_plan": flop_pick["plan"], "flop_sliced": flop_pick["sliced"], "rows": rows_out}
+            "flop_plan": flop_pick["plan"], "flop_sliced": flop_pick["sliced"], "rows": rows_out,
+            "model_plan": (model_pick or best)["plan"], "model_sliced": (model_pick or best)["sliced"],
+            "flop_ms": flop_pick.get("measured_ms"), "model_ms": (model_pick or best).get("measured_ms"),
+            "chosen_ms": best.get("measured_ms")}

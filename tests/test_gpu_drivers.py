@@ -241,3 +241,29 @@ def test_dsl_replay_on_gpu(tmp_path, dtype, ext):
     D.execute_dsl_file(dtype, tl2, store2)
     psi = be.load_tensor_data("result")
     assert rel_err(psi, O.qft_closed_form(n, x)) < RTOL[dtype]
+
+
+def test_fp64_norm_of_full_wavefunctions():
+    """The FP64 norm check: the full wavefunction of a unitary circuit from a
+    basis state has norm 1 (QFT-12, config 1) -- 1e-12 in ComplexF64 -- and a
+    5x5 grid circuit's wavefunction contracted through the planned executor in
+    the bench's precision split keeps its norm to 1e-5, summed in FP64."""
+    from picoquant_hip.executor import compile_plan
+    n, x = 12, 2741
+    tn = P.network_from_gates(n, C.qft_gates(n), backend=P.HIPBackend(np.complex128))
+    P.add_input(tn, "".join(str((x >> q) & 1) for q in range(n)))
+    P.full_wavefunction_contraction(tn, "vector")
+    psi = np.asarray(tn.backend.load_tensor_data("result"), dtype=np.complex128)
+    assert abs(np.vdot(psi, psi).real - 1.0) < 1e-12
+    # every output open: 2^16 amplitudes of a 4x4 grid circuit
+    tn = P.network_from_gates(16, C.grid_circuit_gates(4, 4, 8, 7), backend=P.HIPBackend(np.complex128),
+                              decompose=True)
+    P.add_input(tn, "0" * 16)
+    plan = PL.reconfigure(tn, PL.sweep_plan(tn, PL.grid_site_order(4, 4, "col")), windows=(8,))
+    for prec, tol in (("uniform", 1e-12), ("mixed", 1e-5)):
+        p = compile_plan(tn, plan, precision=prec, small_elems=64)
+        p.execute(True)
+        amps = p.result().reshape(-1)
+        p.close()
+        assert amps.size == 2 ** 16
+        assert abs(np.vdot(amps, amps).real - 1.0) < tol, prec
