@@ -77,6 +77,22 @@ int pq_tensor_free(pq_ctx* ctx, pq_tensor t);
 int pq_contract(pq_ctx* ctx, pq_tensor A, int rankA, const int32_t* ia,
                 pq_tensor B, int rankB, const int32_t* ib, pq_tensor* C);
 
+/* pq_contract writing into an existing result tensor, with no device
+ * allocation during the call.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  C must
+ * be a contiguous tensor of the result's dims (as pq_contract would
+ * create it) sharing no buffer with A or B; it is overwritten.  Split-K and
+ * split-reduction partials go to a device array of PQ_NOALLOC_SCRATCH_BYTES
+ * per GPU, held until the process ends (looked up by pq_ctx_create); a
+ * contraction that needs more scratch returns PQ_ERR_UNSUPPORTED.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (two
+ * streams, two contexts, two threads) race.  With profiling enabled
+ * (pq_profile_enable) the call creates timing events. */
+#define PQ_NOALLOC_SCRATCH_BYTES (32u << 20)
+int pq_contract_noalloc(pq_ctx* ctx, pq_tensor A, int rankA, const int32_t* ia,
+                        pq_tensor B, int rankB, const int32_t* ib, pq_tensor C);
+
 /* permute_tensor (src/backends/interactive.jl:111-115 -> src/layer1.jl:111-114):
  * perm is 1-based as in permutedims.  Zero-copy: the result is a strided view
  * that consumers read through; download/reshape materialise it. */

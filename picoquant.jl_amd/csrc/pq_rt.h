@@ -83,6 +83,7 @@ struct pq_ctx {
   void* scratch = nullptr;  // split-K partials for eager calls
   size_t scratch_bytes = 0;
   void* blas = nullptr;     // rocBLAS handle on this stream (rocSOLVER SVD), created on first use
+  void* noalloc_scratch = nullptr;  // this device's scratch array of pq_contract_noalloc
 };
 
 namespace pqrt {
@@ -104,6 +105,15 @@ void release_blas(pq_ctx* c);
 int resolve_step(pq_ctx* c, StepSpec& s, size_t esize);
 // Scratch bytes the step needs (split-K partials), 0 for the others.
 size_t step_scratch_bytes(const StepSpec& s, size_t esize);
+// Checks the ncon labels of a pairwise contraction (pq_contract's rules)
+// and builds it: Ct gets the result's dims and contiguous strides (no
+// buffer), kt the summed extent, and -- when the result is not empty and kt
+// > 0 -- s the resolved step writing Ct's layout.
+int prepare_contract(pq_ctx* c, const Tensor& A, int ra, const int32_t* ia, const Tensor& B, int rb,
+                     const int32_t* ib, Tensor& Ct, StepSpec& s, int64_t& kt);
+// Address of pq_contract_noalloc's device scratch on `device` (looked up
+// once per device, with that device current); nullptr if unavailable.
+void* noalloc_scratch(int device);
 // Launch one resolved step on the context's stream.
 int launch_step(pq_ctx* c, const void* A, const void* B, void* Out, const StepSpec& s, void* scratch);
 

@@ -313,6 +313,71 @@ int resolve_step(pq_ctx* c, StepSpec& s, size_t esize) {
   return PQ_OK;
 }
 
+int prepare_contract(pq_ctx* c, const Tensor& A, int ra, const int32_t* ia, const Tensor& B, int rb,
+                     const int32_t* ib, Tensor& Ct, StepSpec& s, int64_t& kt) {
+  if (ra != A.rank || rb != B.rank) return fail(c, PQ_ERR_ARG, "label count does not match tensor rank");
+  if ((ra && !ia) || (rb && !ib)) return fail(c, PQ_ERR_ARG, "null label array");
+  for (int i = 0; i < ra; ++i) {
+    if (ia[i] == 0) return fail(c, PQ_ERR_ARG, "label 0 is not allowed");
+    for (int j = i + 1; j < ra; ++j)
+      if (ia[i] == ia[j]) return fail(c, PQ_ERR_ARG, "repeated label in A");
+  }
+  for (int i = 0; i < rb; ++i) {
+    if (ib[i] == 0) return fail(c, PQ_ERR_ARG, "label 0 is not allowed");
+    for (int j = i + 1; j < rb; ++j)
+      if (ib[i] == ib[j]) return fail(c, PQ_ERR_ARG, "repeated label in B");
+  }
+  std::vector<int> a_open, b_open;
+  std::vector<std::pair<int, int>> common;
+  for (int i = 0; i < ra; ++i) {
+    if (ia[i] < 0) {
+      for (int j = 0; j < rb; ++j)
+        if (ib[j] == ia[i]) return fail(c, PQ_ERR_ARG, "open label shared by A and B");
+      a_open.push_back(i);
+    } else {
+      int jj = -1;
+      for (int j = 0; j < rb; ++j)
+        if (ib[j] == ia[i]) jj = j;
+      if (jj < 0) return fail(c, PQ_ERR_ARG, "summed label missing from B");
+      if (A.dims[i] != B.dims[jj]) return fail(c, PQ_ERR_ARG, "summed dimensions differ");
+      common.push_back({i, jj});
+    }
+  }
+  for (int j = 0; j < rb; ++j) {
+    if (ib[j] < 0) {
+      b_open.push_back(j);
+    } else {
+      bool found = false;
+      for (int i = 0; i < ra; ++i)
+        if (ia[i] == ib[j]) found = true;
+      if (!found) return fail(c, PQ_ERR_ARG, "summed label missing from A");
+    }
+  }
+  int rc = (int)(a_open.size() + b_open.size());
+  if (rc > PQ_MAX_RANK) return fail(c, PQ_ERR_UNSUPPORTED, "result rank too large");
+  Ct = Tensor();
+  Ct.rank = rc;
+  int p = 0;
+  for (int i : a_open) Ct.dims[p++] = A.dims[i];
+  for (int j : b_open) Ct.dims[p++] = B.dims[j];
+  set_contiguous(Ct);
+  const int64_t nc = numel(Ct);
+  if (nc >= (1ll << 31)) return fail(c, PQ_ERR_UNSUPPORTED, "result has 2^31 or more elements");
+  kt = 1;
+  for (auto& pr : common) kt *= A.dims[pr.first];
+  if (nc == 0 || kt == 0) return PQ_OK;
+  s = StepSpec();
+  p = 0;
+  for (int i : a_open) s.M.push_back({A.dims[i], A.strides[i], Ct.strides[p++]});
+  for (int j : b_open) s.N.push_back({B.dims[j], B.strides[j], Ct.strides[p++]});
+  for (auto& pr : common) s.K.push_back({A.dims[pr.first], A.strides[pr.first], B.strides[pr.second]});
+  s.M = merge_modes(s.M);
+  s.N = merge_modes(s.N);
+  s.K = merge_modes(s.K);
+  s.L = single_batch();
+  return resolve_step(c, s, c->esize);
+}
+
 size_t step_scratch_bytes(const StepSpec& s, size_t esize) {
   if (s.kc == PQ_KC_GEMM && s.splitk > 1) return (size_t)s.splitk * s.Mt * s.Nt * esize;
   if (s.kc != PQ_KC_REDUCE) return 0;
@@ -482,6 +547,7 @@ int pq_ctx_create(int device, int dtype, pq_ctx** out) {
     delete c;
     return PQ_ERR_HIP;
   }
+  c->noalloc_scratch = noalloc_scratch(device);
   *out = c;
   return PQ_OK;
 }
@@ -633,58 +699,14 @@ int pq_contract(pq_ctx* c, pq_tensor ha, int ra, const int32_t* ia, pq_tensor hb
   Tensor* A = get(c, ha);
   Tensor* B = get(c, hb);
   if (!A || !B) return fail(c, PQ_ERR_NOT_FOUND, "unknown tensor handle");
-  if (ra != A->rank || rb != B->rank) return fail(c, PQ_ERR_ARG, "label count does not match tensor rank");
-  if ((ra && !ia) || (rb && !ib)) return fail(c, PQ_ERR_ARG, "null label array");
-  for (int i = 0; i < ra; ++i) {
-    if (ia[i] == 0) return fail(c, PQ_ERR_ARG, "label 0 is not allowed");
-    for (int j = i + 1; j < ra; ++j)
-      if (ia[i] == ia[j]) return fail(c, PQ_ERR_ARG, "repeated label in A");
-  }
-  for (int i = 0; i < rb; ++i) {
-    if (ib[i] == 0) return fail(c, PQ_ERR_ARG, "label 0 is not allowed");
-    for (int j = i + 1; j < rb; ++j)
-      if (ib[i] == ib[j]) return fail(c, PQ_ERR_ARG, "repeated label in B");
-  }
-  std::vector<int> a_open, b_open;
-  std::vector<std::pair<int, int>> common;
-  for (int i = 0; i < ra; ++i) {
-    if (ia[i] < 0) {
-      for (int j = 0; j < rb; ++j)
-        if (ib[j] == ia[i]) return fail(c, PQ_ERR_ARG, "open label shared by A and B");
-      a_open.push_back(i);
-    } else {
-      int jj = -1;
-      for (int j = 0; j < rb; ++j)
-        if (ib[j] == ia[i]) jj = j;
-      if (jj < 0) return fail(c, PQ_ERR_ARG, "summed label missing from B");
-      if (A->dims[i] != B->dims[jj]) return fail(c, PQ_ERR_ARG, "summed dimensions differ");
-      common.push_back({i, jj});
-    }
-  }
-  for (int j = 0; j < rb; ++j) {
-    if (ib[j] < 0) {
-      b_open.push_back(j);
-    } else {
-      bool found = false;
-      for (int i = 0; i < ra; ++i)
-        if (ia[i] == ib[j]) found = true;
-      if (!found) return fail(c, PQ_ERR_ARG, "summed label missing from A");
-    }
-  }
-  int rc = (int)(a_open.size() + b_open.size());
-  if (rc > PQ_MAX_RANK) return fail(c, PQ_ERR_UNSUPPORTED, "result rank too large");
   Tensor Ct;
-  Ct.rank = rc;
-  int p = 0;
-  for (int i : a_open) Ct.dims[p++] = A->dims[i];
-  for (int j : b_open) Ct.dims[p++] = B->dims[j];
-  set_contiguous(Ct);
-  int64_t nc = numel(Ct);
-  if (nc >= (1ll << 31)) return fail(c, PQ_ERR_UNSUPPORTED, "result has 2^31 or more elements");
+  StepSpec s;
+  int64_t kt = 0;
+  int r = prepare_contract(c, *A, ra, ia, *B, rb, ib, Ct, s, kt);
+  if (r) return r;
+  const int64_t nc = numel(Ct);
   Ct.buf = new_buf(c, (size_t)nc * c->esize);
   if (!Ct.buf) return fail(c, PQ_ERR_OOM, "device allocation failed");
-  int64_t kt = 1;
-  for (auto& pr : common) kt *= A->dims[pr.first];
   if (nc > 0 && kt == 0) {
     hipError_t e = hipMemsetAsync(Ct.buf->ptr, 0, (size_t)nc * c->esize, c->stream);
     if (e != hipSuccess) {
@@ -692,31 +714,19 @@ int pq_contract(pq_ctx* c, pq_tensor ha, int ra, const int32_t* ia, pq_tensor hb
       return hipfail(c, e, "memset");
     }
   } else if (nc > 0) {
-    StepSpec s;
-    p = 0;
-    for (int i : a_open) s.M.push_back({A->dims[i], A->strides[i], Ct.strides[p++]});
-    for (int j : b_open) s.N.push_back({B->dims[j], B->strides[j], Ct.strides[p++]});
-    for (auto& pr : common) s.K.push_back({A->dims[pr.first], A->strides[pr.first], B->strides[pr.second]});
-    s.M = merge_modes(s.M);
-    s.N = merge_modes(s.N);
-    s.K = merge_modes(s.K);
-    s.L = single_batch();
-    int r = resolve_step(c, s, c->esize);
-    if (r == PQ_OK) {
-      size_t need = step_scratch_bytes(s, c->esize);
-      if (need > c->scratch_bytes) {
-        if (c->scratch) {
-          hipStreamSynchronize(c->stream);
-          hipFree(c->scratch);
-          c->scratch = nullptr;
-          c->scratch_bytes = 0;
-        }
-        if (hipMalloc(&c->scratch, need) != hipSuccess) {
-          (void)hipGetLastError();
-          r = fail(c, PQ_ERR_OOM, "scratch allocation failed");
-        } else {
-          c->scratch_bytes = need;
-        }
+    size_t need = step_scratch_bytes(s, c->esize);
+    if (need > c->scratch_bytes) {
+      if (c->scratch) {
+        hipStreamSynchronize(c->stream);
+        hipFree(c->scratch);
+        c->scratch = nullptr;
+        c->scratch_bytes = 0;
+      }
+      if (hipMalloc(&c->scratch, need) != hipSuccess) {
+        (void)hipGetLastError();
+        r = fail(c, PQ_ERR_OOM, "scratch allocation failed");
+      } else {
+        c->scratch_bytes = need;
       }
     }
     if (r == PQ_OK) {

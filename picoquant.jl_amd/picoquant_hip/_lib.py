@@ -52,6 +52,7 @@ SIGNATURES = {
     "pq_tensor_free": (ct.c_int, [_P, ct.c_int64]),
     "pq_tensor_write": (ct.c_int, [_P, ct.c_int64, _P, ct.c_int]),
     "pq_contract": (ct.c_int, [_P, ct.c_int64, ct.c_int, _I32P, ct.c_int64, ct.c_int, _I32P, _I64P]),
+    "pq_contract_noalloc": (ct.c_int, [_P, ct.c_int64, ct.c_int, _I32P, ct.c_int64, ct.c_int, _I32P, ct.c_int64]),
     "pq_permute": (ct.c_int, [_P, ct.c_int64, _I32P, _I64P]),
     "pq_decompose": (ct.c_int, [_P, ct.c_int64, ct.c_int, _I32P, ct.c_int, _I32P, ct.c_double, ct.c_int,
                                 _I64P, _I64P, ct.POINTER(ct.c_int)]),
@@ -210,6 +211,11 @@ class Context:
         self.check(self.lib.pq_contract(self.ptr, ha, len(ia), i32arr(ia), hb, len(ib), i32arr(ib),
                                         ct.byref(out)))
         return out.value
+
+    def contract_into(self, ha, ia, hb, ib, hc):
+        """pq_contract_noalloc: the contraction written into the existing
+        contiguous tensor hc (no device allocation during the call)."""
+        self.check(self.lib.pq_contract_noalloc(self.ptr, ha, len(ia), i32arr(ia), hb, len(ib), i32arr(ib), hc))
 
     def decompose(self, h, left, right, threshold=1e-13, max_rank=0):
         """SVD split (pq_decompose): (B, C, chi) for 1-based axis lists."""
