@@ -54,6 +54,7 @@ struct MultiDesc {
   Group M, N, K;      // as GenArgs
   Batch L;            // batch entries of one slice group (grid z)
   int32_t conv;       // 1: element-type conversion of M.total elements A -> C (the other type)
+  int32_t c128;       // forest launches: the step computes in (conversions: converts from) complex128
   int32_t nhigh;
   int64_t hA[PQ_MAX_HIGH_BITS], hB[PQ_MAX_HIGH_BITS], hC[PQ_MAX_HIGH_BITS];
 };
@@ -61,6 +62,13 @@ struct MultiDesc {
 template <typename C>
 hipError_t launch_multi(const MultiDesc* descs, int n, unsigned blocks_x, unsigned batch, int64_t group,
                         hipStream_t s);
+
+// Independent trees of small unbatched steps in one launch, one workgroup
+// per tree, run level by level with a barrier between levels.  tab[b] is
+// the offset of tree b's entry: nlevels, then per level nsteps, the level's
+// output count and per step (descriptor index, its first output), first
+// outputs ascending from 0.
+hipError_t launch_forest(const MultiDesc* descs, const int32_t* tab, int ntrees, hipStream_t s);
 
 template <typename C>
 hipError_t launch_stream(const C* X, const C* Y, C* Out, const StreamArgs& a, hipStream_t s);

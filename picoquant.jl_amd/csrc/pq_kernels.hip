@@ -247,6 +247,72 @@ __global__ __launch_bounds__(256) void k_multi(const MultiDesc* __restrict__ des
   }
 }
 
+// ------------------------------------------------------------------ k_forest
+// Output element o of one small unbatched step (or element o of an
+// element-type conversion): one lane, loop over K.
+template <typename C>
+__device__ __forceinline__ void forest_output(const MultiDesc& d, uint32_t o) {
+  if (d.conv) {
+    typedef typename std::conditional<sizeof(C) == 16, c64, c128>::type D;
+    const C v = reinterpret_cast<const C*>(d.A)[o];
+    D w;
+    w.x = (decltype(w.x))v.x;
+    w.y = (decltype(w.y))v.y;
+    reinterpret_cast<D*>(d.C)[o] = w;
+    return;
+  }
+  const C* A = reinterpret_cast<const C*>(d.A);
+  const C* B = reinterpret_cast<const C*>(d.B);
+  const uint32_t Mt = d.M.total, Kt = d.K.total;
+  const uint32_t m = o % Mt, n = o / Mt;
+  int32_t pa, pc1, pb, pc2;
+  group_offsets(d.M, m, pa, pc1);
+  group_offsets(d.N, n, pb, pc2);
+  C acc = czero<C>();
+  // the loads of several K values in flight at once (a tree's level waits
+  // for its slowest lane)
+#pragma unroll 4
+  for (uint32_t k = 0; k < Kt; ++k) {
+    int32_t ka, kb;
+    group_offsets(d.K, k, ka, kb);
+    cfma(acc, A[pa + ka], B[pb + kb]);
+  }
+  reinterpret_cast<C*>(d.C)[pc1 + pc2] = acc;
+}
+
+// Workgroup b runs tree b level by level: the outputs of all the level's
+// steps are spread over the lanes (a level's steps read only earlier
+// levels), and the barrier after each level makes its stores visible to the
+// workgroup's later levels (one workgroup runs on one CU).  Table: tab[b] =
+// offset of tree b's entry, which is nlevels, then per level nsteps, its
+// output count, and per step (descriptor, first output), outputs ascending.
+__global__ __launch_bounds__(256) void k_forest(const MultiDesc* __restrict__ descs, const int32_t* __restrict__ tab) {
+  int32_t q = tab[blockIdx.x];
+  const int32_t nlev = tab[q++];
+  for (int32_t l = 0; l < nlev; ++l) {
+    const int32_t ns = tab[q], tot = tab[q + 1];
+    const int32_t* ent = tab + q + 2;
+    for (int32_t t = threadIdx.x; t < tot; t += blockDim.x) {
+      int32_t lo = 0, hi = ns - 1;   // last step whose first output is <= t
+      while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (ent[2 * mid + 1] <= t)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      const MultiDesc& d = descs[ent[2 * lo]];
+      const uint32_t o = (uint32_t)(t - ent[2 * lo + 1]);
+      if (d.c128)
+        forest_output<c128>(d, o);
+      else
+        forest_output<c64>(d, o);
+    }
+    __syncthreads();
+    q += 2 + 2 * ns;
+  }
+}
+
 inline unsigned grid_for(uint64_t work, unsigned block, unsigned cap = 8192) {
   uint64_t g = (work + block - 1) / block;
   if (g < 1) g = 1;
@@ -310,6 +376,12 @@ hipError_t launch_multi(const MultiDesc* descs, int n, unsigned blocks_x, unsign
                         hipStream_t s) {
   if (n < 1 || n > 65535 || batch < 1 || batch > 65535) return hipErrorInvalidValue;
   hipLaunchKernelGGL((k_multi<C>), dim3(blocks_x, n, batch), dim3(256), 0, s, descs, group);
+  return hipGetLastError();
+}
+
+hipError_t launch_forest(const MultiDesc* descs, const int32_t* tab, int ntrees, hipStream_t s) {
+  if (ntrees < 1 || ntrees > (1 << 24)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_forest, dim3(ntrees), dim3(256), 0, s, descs, tab);
   return hipGetLastError();
 }
 

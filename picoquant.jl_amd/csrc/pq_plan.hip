@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <functional>
 
 using namespace pqrt;
 
@@ -134,8 +135,10 @@ struct PStep {
 // small steps of one dependency level (descriptors at desc_off).
 struct Launch {
   bool grouped = false;
+  bool forest = false;     // grouped as independent trees, one workgroup each (k_forest)
   int step = -1;
-  std::vector<int> steps;  // grouped: member steps
+  std::vector<int> steps;  // grouped: member steps (forest: tree after tree, each in dependency order)
+  std::vector<int> tree_start;  // forest: offset of each tree in steps, then steps.size()
   int desc_off = 0;
   int dtype = PQ_C128;
   unsigned blocks_x = 1, batch = 1;
@@ -150,6 +153,7 @@ struct pq_plan {
   std::vector<Launch> seq1, seq2;    // launch sequences of phase 1 / phase 2
   MultiDesc* descs = nullptr;        // device copies of the grouped descriptors
   int32_t* chain_tab = nullptr;      // device copy of the fused-chain address tables
+  int32_t* forest_tab = nullptr;     // device copy of the forest launch's tree table
   int n_sliced = 0, group_bits = 0;
   int64_t n_groups = 1;
   int n_out_bits = 0;                // the top n_out_bits sliced bonds number output batches
@@ -198,6 +202,7 @@ void free_plan(pq_ctx* c, pq_plan* p) {
   if (p->out_dev) hipFree(p->out_dev);
   if (p->descs) hipFree(p->descs);
   if (p->chain_tab) hipFree(p->chain_tab);
+  if (p->forest_tab) hipFree(p->forest_tab);
   delete p;
 }
 
@@ -566,9 +571,14 @@ int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
     hipError_t e;
     {
       ProfScope ps(c, PQ_KC_BATCHED, by, fl);
-      e = (L.dtype == PQ_C128)
-              ? launch_multi<c128>(p->descs + L.desc_off, (int)L.steps.size(), L.blocks_x, L.batch, group, c->stream)
-              : launch_multi<c64>(p->descs + L.desc_off, (int)L.steps.size(), L.blocks_x, L.batch, group, c->stream);
+      if (L.forest)
+        e = launch_forest(p->descs + L.desc_off, p->forest_tab, (int)L.tree_start.size() - 1, c->stream);
+      else
+        e = (L.dtype == PQ_C128)
+                ? launch_multi<c128>(p->descs + L.desc_off, (int)L.steps.size(), L.blocks_x, L.batch, group,
+                                     c->stream)
+                : launch_multi<c64>(p->descs + L.desc_off, (int)L.steps.size(), L.blocks_x, L.batch, group,
+                                    c->stream);
     }
     if (profile_steps) {
       hipEventRecord(p->ev[2 * L.steps[0] + 1], c->stream);
@@ -876,7 +886,64 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
       i = j;
     }
   };
-  build_seq(p->order1, p->seq1);
+  // Phase 1's forest: the small unbatched steps whose operands are leaves or
+  // results of other forest steps -- the world lines, which every qubit
+  // contracts on its own.  They form independent trees (each result has one
+  // reader), which one launch runs with a workgroup per tree, in place of a
+  // grouped launch per dependency level.
+  Launch forest;
+  {
+    std::vector<int> in_forest(d->n_steps, 0);
+    std::vector<int> rest;
+    std::vector<int> o = p->order1;
+    std::stable_sort(o.begin(), o.end(), [&](int x, int y) { return p->steps[x].level < p->steps[y].level; });
+    std::vector<int> fs;
+    // one workgroup runs a whole tree, so only steps of a few thousand
+    // outputs (the world lines' gates and the first merges) join it; larger
+    // small steps keep a grouped launch per level, with many workgroups each
+    constexpr int64_t kForestOut = 2048, kForestK = 16;
+    for (int i : o) {
+      const PStep& st = p->steps[i];
+      bool ok = small(st) && (st.kind == 1 ? conv_elems(st) <= kForestOut
+                                           : st.spec.L.total == 1 && st.spec.Mt * st.spec.Nt <= (uint64_t)kForestOut &&
+                                                 st.spec.Kt <= (uint64_t)kForestK);
+      for (int x : {st.a, st.b}) {
+        const Slot& sl = p->slots[x];
+        if (!sl.leaf && !in_forest[sl.producer]) ok = false;
+        if (sl.leaf && st.kind == 0 && sl.dtype != st.spec.dtype) ok = false;
+      }
+      if (ok) {
+        in_forest[i] = 1;
+        fs.push_back(i);
+      } else {
+        rest.push_back(i);
+      }
+    }
+    if (fs.size() >= 2) {
+      // trees: union of each step with the forest steps that produce its operands
+      std::vector<int> parent(d->n_steps);
+      for (int i = 0; i < d->n_steps; ++i) parent[i] = i;
+      std::function<int(int)> root = [&](int x) { return parent[x] == x ? x : parent[x] = root(parent[x]); };
+      for (int i : fs)
+        for (int x : {p->steps[i].a, p->steps[i].b}) {
+          const Slot& sl = p->slots[x];
+          if (!sl.leaf) parent[root(sl.producer)] = root(i);
+        }
+      std::map<int, std::vector<int>> trees;   // by root; steps in dependency order (fs is level-sorted)
+      for (int i : fs) trees[root(i)].push_back(i);
+      forest.grouped = true;
+      forest.forest = true;
+      for (auto& kv : trees) {
+        forest.tree_start.push_back((int)forest.steps.size());
+        forest.steps.insert(forest.steps.end(), kv.second.begin(), kv.second.end());
+      }
+      forest.tree_start.push_back((int)forest.steps.size());
+      p->seq1.push_back(forest);
+      build_seq(rest, p->seq1);
+    } else {
+      build_seq(p->order1, p->seq1);
+    }
+  }
   build_seq(p->order2, p->seq2);
   // Liveness over the launch sequence [phase 1][phase 2 of one group]; the
   // steps of one launch share a position; anything a phase-2 step reads
@@ -967,6 +1034,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
         const PStep& st = p->steps[i];
         MultiDesc m;
         memset(&m, 0, sizeof(m));
+        m.c128 = p->slots[st.a].dtype == PQ_C128 ? 1 : 0;
         m.A = slot_ptr(c, p, st.a, 0);
         m.B = slot_ptr(c, p, st.b, 0);
         m.C = slot_ptr(c, p, st.c, 0);
@@ -1009,6 +1077,44 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
       (void)hipGetLastError();
       free_plan(c, p);
       return fail(c, PQ_ERR_OOM, "descriptor upload failed");
+    }
+  }
+  for (const Launch& L : p->seq1) {
+    if (!L.forest) continue;
+    // tree table (launch_forest): per tree its levels, per level the steps
+    // (positions in the launch's descriptors) with their first outputs
+    const int nt = (int)L.tree_start.size() - 1;
+    std::vector<int32_t> ft(nt, 0);
+    bool big = false;
+    for (int b = 0; b < nt; ++b) {
+      ft[b] = (int32_t)ft.size();
+      std::map<int, std::vector<int>> lv;   // level -> positions
+      for (int q = L.tree_start[b]; q < L.tree_start[b + 1]; ++q) lv[p->steps[L.steps[q]].level].push_back(q);
+      ft.push_back((int32_t)lv.size());
+      for (auto& kv : lv) {
+        ft.push_back((int32_t)kv.second.size());
+        const size_t tot_at = ft.size();
+        ft.push_back(0);
+        int64_t first = 0;
+        for (int q : kv.second) {
+          const PStep& st = p->steps[L.steps[q]];
+          ft.push_back(q);
+          ft.push_back((int32_t)first);
+          first += st.kind == 1 ? conv_elems(st) : (int64_t)(st.spec.Mt * st.spec.Nt);
+        }
+        if (first >= (1ll << 30)) big = true;
+        ft[tot_at] = (int32_t)first;
+      }
+    }
+    if (big) {
+      free_plan(c, p);
+      return fail(c, PQ_ERR_UNSUPPORTED, "forest level too large");
+    }
+    if (hipMalloc((void**)&p->forest_tab, ft.size() * sizeof(int32_t)) != hipSuccess ||
+        hipMemcpy(p->forest_tab, ft.data(), ft.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipGetLastError();
+      free_plan(c, p);
+      return fail(c, PQ_ERR_OOM, "forest table upload failed");
     }
   }
   if (!chain_tab.empty()) {

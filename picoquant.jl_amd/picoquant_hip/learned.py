@@ -62,7 +62,11 @@ def candidate_plans(tn, rows, cols, open_q, n_candidates=64, seed=0, n_greedy=2,
         swaps = 1 + (k // 2) % 6
         cands.append((f"sweep-{name}-p{k}", PL.sweep_plan(tn, _perturbed(base[name], rng, swaps, open_q))))
         k += 1
-    return cands[:n_candidates]
+    out = cands[:n_candidates]
+    if not any(name == "bgreedy-tau0" for name, _ in out):
+        # the greedy plan is the selector's fallback: it is always a candidate
+        out[-1] = ("bgreedy-tau0", plan0)
+    return out
 
 
 def select_plan(tn, rows, cols, open_q, n_sliced, model=None, n_candidates=64, shortlist=8,
