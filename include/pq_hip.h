@@ -281,6 +281,15 @@ int pq_plan_step_info(const pq_plan* plan, int step, int* kclass, int* tile, int
  * summed over all slice groups. */
 int pq_plan_profile(pq_ctx* ctx, pq_plan* plan, double* step_ms);
 int pq_plan_destroy(pq_ctx* ctx, pq_plan* plan);
+/* Streams a plan created afterwards spreads independent launches over: the
+ * context's stream plus extra streams of default priority, with launches
+ * that share no data and no reused arena region on different streams (a
+ * captured graph then holds them as parallel branches).  Only n = 1 (the
+ * default, every launch on the context's stream) is accepted in this build:
+ * n > 1 returns PQ_ERR_UNSUPPORTED -- a test run with four streams failed
+ * mixed-precision plans and ended in a host segmentation fault whose cause
+ * is not found. */
+int pq_set_plan_streams(pq_ctx* ctx, int n);
 
 /* ------------------------------------------------------------------------
  * Cost model and autotuning (no counterpart in the reference, whose planners

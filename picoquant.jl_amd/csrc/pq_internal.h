@@ -63,12 +63,23 @@ template <typename C>
 hipError_t launch_multi(const MultiDesc* descs, int n, unsigned blocks_x, unsigned batch, int64_t group,
                         hipStream_t s);
 
-// Independent trees of small unbatched steps in one launch, one workgroup
-// per tree, run level by level with a barrier between levels.  tab[b] is
-// the offset of tree b's entry: nlevels, then per level nsteps, the level's
-// output count and per step (descriptor index, its first output), first
-// outputs ascending from 0.
-hipError_t launch_forest(const MultiDesc* descs, const int32_t* tab, int ntrees, hipStream_t s);
+// Independent trees of small unbatched steps in one launch (k_forest), one
+// workgroup per tree, run level by level with a barrier between levels.
+// Every output element of a step is one item (step, A offset, B offset,
+// C offset), precomputed on the host; a step's K offsets (A, B) are
+// ktab[kt .. kt + nk).  Conversions: item = (step, source offset, -, target
+// offset).  lv[b] is the offset in lv of tree b's entry: nlevels, then the
+// nlevels + 1 item boundaries of its levels.
+struct ForestStep {
+  const void* A;
+  const void* B;
+  void* C;
+  int32_t nk, kt;     // K extent, first K table entry
+  int32_t conv;       // 1: element-type conversion A -> C (the other type)
+  int32_t c128;       // computes in (converts from) complex128
+};
+hipError_t launch_forest(const ForestStep* steps, const int4* items, const int2* ktab, const int32_t* lv,
+                         int ntrees, hipStream_t s);
 
 template <typename C>
 hipError_t launch_stream(const C* X, const C* Y, C* Out, const StreamArgs& a, hipStream_t s);

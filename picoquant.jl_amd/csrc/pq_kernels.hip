@@ -248,68 +248,52 @@ __global__ __launch_bounds__(256) void k_multi(const MultiDesc* __restrict__ des
 }
 
 // ------------------------------------------------------------------ k_forest
-// Output element o of one small unbatched step (or element o of an
-// element-type conversion): one lane, loop over K.
+// One item: an output element of a small unbatched step (one lane, the K
+// offsets from the step's table), or one element of a conversion.
 template <typename C>
-__device__ __forceinline__ void forest_output(const MultiDesc& d, uint32_t o) {
-  if (d.conv) {
+__device__ __forceinline__ void forest_item(const ForestStep& f, const int4& it, const int2* __restrict__ ktab) {
+  if (f.conv) {
     typedef typename std::conditional<sizeof(C) == 16, c64, c128>::type D;
-    const C v = reinterpret_cast<const C*>(d.A)[o];
+    const C v = reinterpret_cast<const C*>(f.A)[it.y];
     D w;
     w.x = (decltype(w.x))v.x;
     w.y = (decltype(w.y))v.y;
-    reinterpret_cast<D*>(d.C)[o] = w;
+    reinterpret_cast<D*>(f.C)[it.w] = w;
     return;
   }
-  const C* A = reinterpret_cast<const C*>(d.A);
-  const C* B = reinterpret_cast<const C*>(d.B);
-  const uint32_t Mt = d.M.total, Kt = d.K.total;
-  const uint32_t m = o % Mt, n = o / Mt;
-  int32_t pa, pc1, pb, pc2;
-  group_offsets(d.M, m, pa, pc1);
-  group_offsets(d.N, n, pb, pc2);
+  const C* A = reinterpret_cast<const C*>(f.A) + it.y;
+  const C* B = reinterpret_cast<const C*>(f.B) + it.z;
   C acc = czero<C>();
   // the loads of several K values in flight at once (a tree's level waits
   // for its slowest lane)
 #pragma unroll 4
-  for (uint32_t k = 0; k < Kt; ++k) {
-    int32_t ka, kb;
-    group_offsets(d.K, k, ka, kb);
-    cfma(acc, A[pa + ka], B[pb + kb]);
+  for (int32_t k = 0; k < f.nk; ++k) {
+    const int2 o = ktab[f.kt + k];
+    cfma(acc, A[o.x], B[o.y]);
   }
-  reinterpret_cast<C*>(d.C)[pc1 + pc2] = acc;
+  reinterpret_cast<C*>(f.C)[it.w] = acc;
 }
 
-// Workgroup b runs tree b level by level: the outputs of all the level's
-// steps are spread over the lanes (a level's steps read only earlier
-// levels), and the barrier after each level makes its stores visible to the
-// workgroup's later levels (one workgroup runs on one CU).  Table: tab[b] =
-// offset of tree b's entry, which is nlevels, then per level nsteps, its
-// output count, and per step (descriptor, first output), outputs ascending.
-__global__ __launch_bounds__(256) void k_forest(const MultiDesc* __restrict__ descs, const int32_t* __restrict__ tab) {
-  int32_t q = tab[blockIdx.x];
-  const int32_t nlev = tab[q++];
+// Workgroup b runs tree b level by level: a level's items (the outputs of
+// all its steps, which read only earlier levels) are spread over the lanes,
+// and the barrier after each level makes its stores visible to the
+// workgroup's later levels (one workgroup runs on one CU).
+__global__ __launch_bounds__(256) void k_forest(const ForestStep* __restrict__ fs, const int4* __restrict__ items,
+                                                const int2* __restrict__ ktab, const int32_t* __restrict__ lv) {
+  const int32_t q = lv[blockIdx.x];
+  const int32_t nlev = lv[q];
+  const int32_t* bnd = lv + q + 1;
   for (int32_t l = 0; l < nlev; ++l) {
-    const int32_t ns = tab[q], tot = tab[q + 1];
-    const int32_t* ent = tab + q + 2;
-    for (int32_t t = threadIdx.x; t < tot; t += blockDim.x) {
-      int32_t lo = 0, hi = ns - 1;   // last step whose first output is <= t
-      while (lo < hi) {
-        const int32_t mid = (lo + hi + 1) >> 1;
-        if (ent[2 * mid + 1] <= t)
-          lo = mid;
-        else
-          hi = mid - 1;
-      }
-      const MultiDesc& d = descs[ent[2 * lo]];
-      const uint32_t o = (uint32_t)(t - ent[2 * lo + 1]);
-      if (d.c128)
-        forest_output<c128>(d, o);
+    const int32_t i1 = bnd[l + 1];
+    for (int32_t i = bnd[l] + (int32_t)threadIdx.x; i < i1; i += (int32_t)blockDim.x) {
+      const int4 it = items[i];
+      const ForestStep& f = fs[it.x];
+      if (f.c128)
+        forest_item<c128>(f, it, ktab);
       else
-        forest_output<c64>(d, o);
+        forest_item<c64>(f, it, ktab);
     }
     __syncthreads();
-    q += 2 + 2 * ns;
   }
 }
 
@@ -379,9 +363,10 @@ hipError_t launch_multi(const MultiDesc* descs, int n, unsigned blocks_x, unsign
   return hipGetLastError();
 }
 
-hipError_t launch_forest(const MultiDesc* descs, const int32_t* tab, int ntrees, hipStream_t s) {
+hipError_t launch_forest(const ForestStep* steps, const int4* items, const int2* ktab, const int32_t* lv,
+                         int ntrees, hipStream_t s) {
   if (ntrees < 1 || ntrees > (1 << 24)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_forest, dim3(ntrees), dim3(256), 0, s, descs, tab);
+  hipLaunchKernelGGL(k_forest, dim3(ntrees), dim3(256), 0, s, steps, items, ktab, lv);
   return hipGetLastError();
 }
 

@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cstring>
 #include <functional>
+#include <set>
 
 using namespace pqrt;
 
@@ -115,6 +116,7 @@ struct Slot {
   int64_t leaf_span = 0;              // leaves: largest element offset the tensor's view reaches
   std::vector<int64_t> slice_stride;  // unbatched slots: stride of each sliced bond kept (0 = absent)
   int64_t arena_off = -1;             // intermediates (bytes)
+  size_t region = 0;                  // intermediates: bytes of the arena region (all copies)
   int producer = -1;
   int last_use = -1;
 };
@@ -144,6 +146,15 @@ struct Launch {
   unsigned blocks_x = 1, batch = 1;
 };
 
+// Where a launch of a phase's sequence runs: its stream, the launches of the
+// same sequence it waits for on other streams, and whether a later launch
+// waits for it (an event is recorded after it).
+struct Sched {
+  int stream = 0;
+  std::vector<int> waits;
+  bool record = false;
+};
+
 }  // namespace
 
 struct pq_plan {
@@ -153,7 +164,11 @@ struct pq_plan {
   std::vector<Launch> seq1, seq2;    // launch sequences of phase 1 / phase 2
   MultiDesc* descs = nullptr;        // device copies of the grouped descriptors
   int32_t* chain_tab = nullptr;      // device copy of the fused-chain address tables
-  int32_t* forest_tab = nullptr;     // device copy of the forest launch's tree table
+  void* forest_buf = nullptr;        // the forest launch's tables (steps, items, K offsets, levels)
+  const ForestStep* f_steps = nullptr;
+  const int4* f_items = nullptr;
+  const int2* f_ktab = nullptr;
+  const int32_t* f_lv = nullptr;
   int n_sliced = 0, group_bits = 0;
   int64_t n_groups = 1;
   int n_out_bits = 0;                // the top n_out_bits sliced bonds number output batches
@@ -169,6 +184,10 @@ struct pq_plan {
   hipGraphExec_t exec = nullptr;
   int64_t launches = 0;
   double alg_flops = 0, alg_bytes = 0;
+  int nstreams = 1;                  // streams the launches spread over (1 = all on the context's)
+  std::vector<Sched> sch[2];         // per launch of seq1 / seq2
+  std::vector<hipEvent_t> lev[2];    // recorded after the launches others wait for
+  hipEvent_t fork_ev[2] = {}, join_ev[2][PQ_PLAN_STREAMS] = {};
   std::vector<hipEvent_t> ev;  // per-step event pairs for pq_plan_profile
   std::vector<char> ev_rec;    // which pairs the last profiled run recorded
 };
@@ -202,8 +221,105 @@ void free_plan(pq_ctx* c, pq_plan* p) {
   if (p->out_dev) hipFree(p->out_dev);
   if (p->descs) hipFree(p->descs);
   if (p->chain_tab) hipFree(p->chain_tab);
-  if (p->forest_tab) hipFree(p->forest_tab);
+  if (p->forest_buf) hipFree(p->forest_buf);
+  for (auto& v : p->lev)
+    for (hipEvent_t e : v) hipEventDestroy(e);
+  for (int ph = 0; ph < 2; ++ph) {
+    if (p->fork_ev[ph]) hipEventDestroy(p->fork_ev[ph]);
+    for (hipEvent_t e : p->join_ev[ph])
+      if (e) hipEventDestroy(e);
+  }
   delete p;
+}
+
+// Slots launch L reads and writes, and whether it uses the plan's split-K
+// scratch.
+void launch_slots(const pq_plan* p, const Launch& L, std::vector<int>& rd, std::vector<int>& wr, bool& scratch,
+                  size_t esize_ctx) {
+  rd.clear();
+  wr.clear();
+  scratch = false;
+  const std::vector<int> one{L.step};
+  for (int i : (L.grouped ? L.steps : one)) {
+    const PStep& st = p->steps[i];
+    rd.push_back(st.a);
+    if (st.kind == 0) rd.push_back(st.b);
+    if (st.kind == 2) rd.insert(rd.end(), st.ops.begin(), st.ops.end());
+    wr.push_back(st.c);
+    if (!L.grouped && st.kind == 0 && step_scratch_bytes(st.spec, esize_of(st.spec.dtype)) > 0) scratch = true;
+  }
+  (void)esize_ctx;
+}
+
+// Schedule of one phase's launch sequence over nstreams streams.  Launch t
+// waits for (1) the launches of the sequence that produce what it reads,
+// (2) every earlier launch of the sequence that read or wrote an arena slot
+// whose region its outputs reuse (the liveness pass only keeps regions of
+// slots live at the same position apart), and (3) the previous user of the
+// split-K scratch.  Each launch goes on the stream whose last launch is its
+// latest dependency (no wait needed for it), else on the least recently used
+// stream; the other dependencies become event waits, one per stream (the
+// latest: a stream runs its launches in order).
+void make_schedule(const pq_plan* p, const std::vector<Launch>& seq, int nstreams, std::vector<Sched>& out) {
+  const int n = (int)seq.size();
+  out.assign(n, Sched());
+  std::map<int, int> prod;                  // slot -> producing launch
+  std::map<int, std::vector<int>> touch;    // arena slot -> launches reading or writing it
+  std::vector<int> rd, wr, last_on(nstreams, -1);
+  int last_scratch = -1;
+  for (int t = 0; t < n; ++t) {
+    bool scr = false;
+    launch_slots(p, seq[t], rd, wr, scr, 0);
+    std::set<int> deps;
+    for (int x : rd) {
+      auto it = prod.find(x);
+      if (it != prod.end() && it->second != t) deps.insert(it->second);
+    }
+    for (int cs : wr) {
+      const Slot& C = p->slots[cs];
+      if (C.arena_off < 0) continue;
+      const size_t c0 = (size_t)C.arena_off, c1 = c0 + C.region;
+      for (auto& kv : touch) {
+        if (kv.first == cs) continue;
+        const Slot& X = p->slots[kv.first];
+        const size_t x0 = (size_t)X.arena_off, x1 = x0 + X.region;
+        if (x0 < c1 && c0 < x1)
+          for (int u : kv.second)
+            if (u != t) deps.insert(u);
+      }
+    }
+    if (scr) {
+      if (last_scratch >= 0) deps.insert(last_scratch);
+      last_scratch = t;
+    }
+    // stream: the one whose last launch is the latest dependency, if any
+    int s = -1, best = -1;
+    for (int u : deps)
+      if (u > best && last_on[out[u].stream] == u) {
+        best = u;
+        s = out[u].stream;
+      }
+    if (s < 0) {
+      s = 0;
+      for (int q = 1; q < nstreams; ++q)
+        if (last_on[q] < last_on[s]) s = q;
+    }
+    out[t].stream = s;
+    std::vector<int> latest(nstreams, -1);
+    for (int u : deps)
+      if (out[u].stream != s) latest[out[u].stream] = std::max(latest[out[u].stream], u);
+    for (int q = 0; q < nstreams; ++q)
+      if (latest[q] >= 0) {
+        out[t].waits.push_back(latest[q]);
+        out[latest[q]].record = true;
+      }
+    last_on[s] = t;
+    for (int x : wr) prod[x] = t;
+    for (int x : rd)
+      if (p->slots[x].arena_off >= 0) touch[x].push_back(t);
+    for (int x : wr)
+      if (p->slots[x].arena_off >= 0) touch[x].push_back(t);
+  }
 }
 
 // Whether `span` (largest element offset read or written) stays inside slot s.
@@ -572,7 +688,7 @@ int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
     {
       ProfScope ps(c, PQ_KC_BATCHED, by, fl);
       if (L.forest)
-        e = launch_forest(p->descs + L.desc_off, p->forest_tab, (int)L.tree_start.size() - 1, c->stream);
+        e = launch_forest(p->f_steps, p->f_items, p->f_ktab, p->f_lv, (int)L.tree_start.size() - 1, c->stream);
       else
         e = (L.dtype == PQ_C128)
                 ? launch_multi<c128>(p->descs + L.desc_off, (int)L.steps.size(), L.blocks_x, L.batch, group,
@@ -587,15 +703,48 @@ int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
     if (e != hipSuccess) return hipfail(c, e, "grouped kernel");
     return PQ_OK;
   };
-  for (const Launch& L : p->seq1) {
-    int r = do_launch(L, 0);
+  // Launches spread over the schedule's streams (not when timing steps):
+  // each phase (and each slice group of phase 2) forks the extra streams off
+  // the context's stream and joins them back, so a captured graph holds the
+  // independent launches as parallel branches.
+  const bool conc = !profile_steps && p->nstreams > 1;
+  hipStream_t main = c->stream;
+  auto strm = [&](int q) { return q == 0 ? main : c->aux[q - 1]; };
+  auto run_seq = [&](int ph, const std::vector<Launch>& seq, int64_t group) -> int {
+    if (conc) {
+      hipEventRecord(p->fork_ev[ph], main);
+      for (int q = 1; q < p->nstreams; ++q) hipStreamWaitEvent(strm(q), p->fork_ev[ph], 0);
+    }
+    for (size_t t = 0; t < seq.size(); ++t) {
+      if (conc) {
+        const Sched& sc = p->sch[ph][t];
+        for (int u : sc.waits) hipStreamWaitEvent(strm(sc.stream), p->lev[ph][u], 0);
+        c->stream = strm(sc.stream);
+      }
+      const int r = do_launch(seq[t], group);
+      if (conc) {
+        if (p->sch[ph][t].record) hipEventRecord(p->lev[ph][t], c->stream);
+        c->stream = main;
+      }
+      if (r) return r;
+    }
+    if (conc)
+      for (int q = 1; q < p->nstreams; ++q) {
+        hipEventRecord(p->join_ev[ph][q], strm(q));
+        hipStreamWaitEvent(main, p->join_ev[ph][q], 0);
+      }
+    if (hipGetLastError() != hipSuccess) return fail(c, PQ_ERR_HIP, "stream scheduling");
+    return PQ_OK;
+  };
+  {
+    int r = run_seq(0, p->seq1, 0);
     if (r) return r;
   }
   const Slot& os = p->slots[p->out_slot];
   int nbatch = (int)os.copies();
   for (int64_t grp = 0; grp < p->n_groups; ++grp) {
-    for (const Launch& L : p->seq2) {
-      int r = do_launch(L, grp);
+    {
+      int r = run_seq(1, p->seq2, grp);
       if (r) return r;
     }
     const void* src = slot_ptr(c, p, p->out_slot, grp);
@@ -1001,6 +1150,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
       off = std::max(off, u.first + u.second);
     }
     p->slots[L.s].arena_off = (int64_t)off;
+    p->slots[L.s].region = L.bytes;
     top = std::max(top, off + L.bytes);
     active.push_back(L);
   }
@@ -1026,7 +1176,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
   std::vector<MultiDesc> hd;
   for (auto* seq : {&p->seq1, &p->seq2}) {
     for (Launch& L : *seq) {
-      if (!L.grouped) continue;
+      if (!L.grouped || L.forest) continue;
       L.desc_off = (int)hd.size();
       uint64_t maxw = 1;
       unsigned maxb = 1;
@@ -1081,41 +1231,89 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
   }
   for (const Launch& L : p->seq1) {
     if (!L.forest) continue;
-    // tree table (launch_forest): per tree its levels, per level the steps
-    // (positions in the launch's descriptors) with their first outputs
+    // Forest tables: a record per step, an item per output element (offsets
+    // decoded here from the step's modes), the K offsets, and per tree its
+    // levels' item ranges.
+    auto decode = [](const std::vector<Mode>& ms, int64_t i, int64_t& o0, int64_t& o1) {
+      o0 = o1 = 0;
+      for (const Mode& m : ms) {
+        const int64_t r = i % m.ext;
+        i /= m.ext;
+        o0 += r * m.s0;
+        o1 += r * m.s1;
+      }
+    };
+    std::vector<ForestStep> fsv;
+    std::vector<int4> items;
+    std::vector<int2> ktab;
     const int nt = (int)L.tree_start.size() - 1;
-    std::vector<int32_t> ft(nt, 0);
-    bool big = false;
+    std::vector<int32_t> lv(nt, 0);
     for (int b = 0; b < nt; ++b) {
-      ft[b] = (int32_t)ft.size();
-      std::map<int, std::vector<int>> lv;   // level -> positions
-      for (int q = L.tree_start[b]; q < L.tree_start[b + 1]; ++q) lv[p->steps[L.steps[q]].level].push_back(q);
-      ft.push_back((int32_t)lv.size());
-      for (auto& kv : lv) {
-        ft.push_back((int32_t)kv.second.size());
-        const size_t tot_at = ft.size();
-        ft.push_back(0);
-        int64_t first = 0;
-        for (int q : kv.second) {
-          const PStep& st = p->steps[L.steps[q]];
-          ft.push_back(q);
-          ft.push_back((int32_t)first);
-          first += st.kind == 1 ? conv_elems(st) : (int64_t)(st.spec.Mt * st.spec.Nt);
+      lv[b] = (int32_t)lv.size();
+      std::map<int, std::vector<int>> lvl;   // level -> steps
+      for (int q = L.tree_start[b]; q < L.tree_start[b + 1]; ++q)
+        lvl[p->steps[L.steps[q]].level].push_back(L.steps[q]);
+      lv.push_back((int32_t)lvl.size());
+      lv.push_back((int32_t)items.size());
+      for (auto& kv : lvl) {
+        for (int i : kv.second) {
+          const PStep& st = p->steps[i];
+          ForestStep f;
+          memset(&f, 0, sizeof(f));
+          f.A = slot_ptr(c, p, st.a, 0);
+          f.B = slot_ptr(c, p, st.b, 0);
+          f.C = slot_ptr(c, p, st.c, 0);
+          f.c128 = p->slots[st.a].dtype == PQ_C128 ? 1 : 0;
+          const int32_t si = (int32_t)fsv.size();
+          if (st.kind == 1) {
+            f.conv = 1;
+            for (int64_t o = 0; o < conv_elems(st); ++o) items.push_back(make_int4(si, (int32_t)o, 0, (int32_t)o));
+          } else {
+            f.kt = (int32_t)ktab.size();
+            f.nk = (int32_t)st.spec.Kt;
+            for (int64_t k = 0; k < (int64_t)st.spec.Kt; ++k) {
+              int64_t ka, kb;
+              decode(st.spec.K, k, ka, kb);
+              ktab.push_back(make_int2((int32_t)ka, (int32_t)kb));
+            }
+            const int64_t mt = (int64_t)st.spec.Mt;
+            for (int64_t o = 0; o < (int64_t)(st.spec.Mt * st.spec.Nt); ++o) {
+              int64_t pa, pc1, pb, pc2;
+              decode(st.spec.M, o % mt, pa, pc1);
+              decode(st.spec.N, o / mt, pb, pc2);
+              items.push_back(make_int4(si, (int32_t)pa, (int32_t)pb, (int32_t)(pc1 + pc2)));
+            }
+          }
+          fsv.push_back(f);
         }
-        if (first >= (1ll << 30)) big = true;
-        ft[tot_at] = (int32_t)first;
+        lv.push_back((int32_t)items.size());
       }
     }
-    if (big) {
+    if (items.size() >= ((size_t)1 << 31)) {
       free_plan(c, p);
-      return fail(c, PQ_ERR_UNSUPPORTED, "forest level too large");
+      return fail(c, PQ_ERR_UNSUPPORTED, "forest too large");
     }
-    if (hipMalloc((void**)&p->forest_tab, ft.size() * sizeof(int32_t)) != hipSuccess ||
-        hipMemcpy(p->forest_tab, ft.data(), ft.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
+    if (ktab.empty()) ktab.push_back(make_int2(0, 0));
+    const size_t b_steps = fsv.size() * sizeof(ForestStep), b_items = items.size() * sizeof(int4),
+                 b_k = ktab.size() * sizeof(int2), b_lv = lv.size() * sizeof(int32_t);
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    const size_t o_items = up(b_steps), o_k = o_items + up(b_items), o_lv = o_k + up(b_k), total = o_lv + up(b_lv);
+    std::vector<unsigned char> blob(total, 0);
+    memcpy(blob.data(), fsv.data(), b_steps);
+    memcpy(blob.data() + o_items, items.data(), b_items);
+    memcpy(blob.data() + o_k, ktab.data(), b_k);
+    memcpy(blob.data() + o_lv, lv.data(), b_lv);
+    if (hipMalloc(&p->forest_buf, total) != hipSuccess ||
+        hipMemcpy(p->forest_buf, blob.data(), total, hipMemcpyHostToDevice) != hipSuccess) {
       (void)hipGetLastError();
       free_plan(c, p);
       return fail(c, PQ_ERR_OOM, "forest table upload failed");
     }
+    unsigned char* fb = (unsigned char*)p->forest_buf;
+    p->f_steps = (const ForestStep*)fb;
+    p->f_items = (const int4*)(fb + o_items);
+    p->f_ktab = (const int2*)(fb + o_k);
+    p->f_lv = (const int32_t*)(fb + o_lv);
   }
   if (!chain_tab.empty()) {
     if (hipMalloc((void**)&p->chain_tab, chain_tab.size() * sizeof(int32_t)) != hipSuccess ||
@@ -1127,6 +1325,28 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     }
   }
   p->launches = 1 + (int64_t)p->seq1.size() + p->n_groups * ((int64_t)p->seq2.size() + 1);
+  // Streams: independent launches of a phase run concurrently
+  // (pq_set_plan_streams; 1 = every launch on the context's stream).
+  p->nstreams = std::max(1, std::min(c->plan_streams, PQ_PLAN_STREAMS));
+  if (p->nstreams > 1 && ensure_aux_streams(c) != PQ_OK) p->nstreams = 1;
+  if (p->nstreams > 1) {
+    const std::vector<Launch>* seqs[2] = {&p->seq1, &p->seq2};
+    bool ok = true;
+    for (int ph = 0; ph < 2 && ok; ++ph) {
+      make_schedule(p, *seqs[ph], p->nstreams, p->sch[ph]);
+      p->lev[ph].assign(seqs[ph]->size(), nullptr);
+      for (size_t t = 0; t < seqs[ph]->size() && ok; ++t)
+        if (p->sch[ph][t].record) ok = hipEventCreateWithFlags(&p->lev[ph][t], hipEventDisableTiming) == hipSuccess;
+      ok = ok && hipEventCreateWithFlags(&p->fork_ev[ph], hipEventDisableTiming) == hipSuccess;
+      for (int q = 1; q < p->nstreams && ok; ++q)
+        ok = hipEventCreateWithFlags(&p->join_ev[ph][q], hipEventDisableTiming) == hipSuccess;
+    }
+    if (!ok) {
+      (void)hipGetLastError();
+      free_plan(c, p);
+      return fail(c, PQ_ERR_HIP, "event creation failed");
+    }
+  }
   *out = p;
   return PQ_OK;
 }
@@ -1224,9 +1444,22 @@ int pq_plan_profile(pq_ctx* c, pq_plan* p, double* step_ms) {
   return PQ_OK;
 }
 
+int pq_set_plan_streams(pq_ctx* c, int n) {
+  if (!c) return PQ_ERR_ARG;
+  if (n < 1 || n > PQ_PLAN_STREAMS) return fail(c, PQ_ERR_ARG, "plan streams out of range");
+  // Concurrent plan runs are switched off: a test run of plans on four
+  // streams failed mixed-precision plans and then ended in a host
+  // segmentation fault inside pq_plan_execute, with the cause not found.
+  if (n > 1) return fail(c, PQ_ERR_UNSUPPORTED, "concurrent plan streams are switched off in this build");
+  c->plan_streams = n;
+  return PQ_OK;
+}
+
 int pq_plan_destroy(pq_ctx* c, pq_plan* p) {
   if (!c || !p) return PQ_ERR_ARG;
   hipStreamSynchronize(c->stream);
+  for (hipStream_t a : c->aux)
+    if (a) hipStreamSynchronize(a);
   if (p->exec) hipGraphExecDestroy(p->exec);
   if (p->graph) hipGraphDestroy(p->graph);
   for (auto& e : p->ev) hipEventDestroy(e);

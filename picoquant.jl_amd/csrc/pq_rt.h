@@ -10,6 +10,8 @@
 #include <unordered_map>
 #include <vector>
 
+#define PQ_PLAN_STREAMS 4   // streams a planned run spreads independent launches over
+
 namespace pqrt {
 
 struct Buf {
@@ -84,6 +86,10 @@ struct pq_ctx {
   size_t scratch_bytes = 0;
   void* blas = nullptr;     // rocBLAS handle on this stream (rocSOLVER SVD), created on first use
   void* noalloc_scratch = nullptr;  // this device's scratch array of pq_contract_noalloc
+  // Extra streams for concurrent branches of a planned run (default priority,
+  // created on first use); stream 0 of a plan's schedule is `stream`.
+  hipStream_t aux[PQ_PLAN_STREAMS - 1] = {};
+  int plan_streams = 1;   // read when a plan is created (above 1 is switched off: pq_set_plan_streams)
 };
 
 namespace pqrt {
@@ -97,6 +103,8 @@ void unref(pq_ctx* c, Buf* b);
 Tensor* get(pq_ctx* c, pq_tensor h);
 pq_tensor add(pq_ctx* c, const Tensor& t);
 int materialize_into(pq_ctx* c, const Tensor& src, Tensor& dst);
+// Creates the context's extra streams if they do not exist yet.
+int ensure_aux_streams(pq_ctx* c);
 // Destroys the context's rocBLAS handle, if one was created.
 void release_blas(pq_ctx* c);
 

@@ -126,6 +126,15 @@ void dev_release(pq_ctx* c, void* p, size_t bytes) {
   c->cached += bytes;
 }
 
+int ensure_aux_streams(pq_ctx* c) {
+  for (hipStream_t& a : c->aux)
+    if (!a && hipStreamCreateWithFlags(&a, hipStreamNonBlocking) != hipSuccess) {
+      a = nullptr;
+      return fail(c, PQ_ERR_HIP, "stream creation failed");
+    }
+  return PQ_OK;
+}
+
 Buf* new_buf(pq_ctx* c, size_t bytes) {
   size_t got = 0;
   void* p = dev_alloc(c, bytes ? bytes : 1, &got);
@@ -561,6 +570,12 @@ int pq_ctx_destroy(pq_ctx* c) {
   for (auto& kv : c->free_blocks) hipFree(kv.second);
   c->free_blocks.clear();
   if (c->scratch) hipFree(c->scratch);
+  for (hipStream_t& a : c->aux)
+    if (a) {
+      hipStreamSynchronize(a);
+      hipStreamDestroy(a);
+      a = nullptr;
+    }
   release_blas(c);
   for (auto& p : c->pe)
     for (auto& ev : p.events) {

@@ -53,6 +53,7 @@ SIGNATURES = {
     "pq_tensor_write": (ct.c_int, [_P, ct.c_int64, _P, ct.c_int]),
     "pq_contract": (ct.c_int, [_P, ct.c_int64, ct.c_int, _I32P, ct.c_int64, ct.c_int, _I32P, _I64P]),
     "pq_contract_noalloc": (ct.c_int, [_P, ct.c_int64, ct.c_int, _I32P, ct.c_int64, ct.c_int, _I32P, ct.c_int64]),
+    "pq_set_plan_streams": (ct.c_int, [_P, ct.c_int]),
     "pq_permute": (ct.c_int, [_P, ct.c_int64, _I32P, _I64P]),
     "pq_decompose": (ct.c_int, [_P, ct.c_int64, ct.c_int, _I32P, ct.c_int, _I32P, ct.c_double, ct.c_int,
                                 _I64P, _I64P, ct.POINTER(ct.c_int)]),
@@ -211,6 +212,10 @@ class Context:
         self.check(self.lib.pq_contract(self.ptr, ha, len(ia), i32arr(ia), hb, len(ib), i32arr(ib),
                                         ct.byref(out)))
         return out.value
+
+    def set_plan_streams(self, n):
+        """Streams that plans created afterwards spread their launches over."""
+        self.check(self.lib.pq_set_plan_streams(self.ptr, int(n)))
 
     def contract_into(self, ha, ia, hb, ib, hc):
         """pq_contract_noalloc: the contraction written into the existing

@@ -219,3 +219,65 @@ def test_plan_on_a_network_sliced_by_the_reference_driver():
         want = np.asarray(t2.backend.load_tensor_data("result")).reshape(-1, order="F")
         # a slice can close off amplitudes exactly: compare against the largest
         assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
+
+
+@pytest.mark.skip(reason="concurrent plan streams are switched off: a run of this test's build with four "
+                         "streams failed mixed-precision plans and ended in a host segmentation fault inside "
+                         "pq_plan_execute, cause not found")
+@pytest.mark.parametrize("case", ["grid5x5_sliced", "grid4x4_chains_bitstrings"])
+def test_concurrent_schedule_matches_one_stream(golden_dir, case):
+    """pq_set_plan_streams: a plan whose independent launches run on four
+    streams (parallel branches of the captured graph) gives the same bits as
+    the same plan on one stream, eager and replayed, and the right amplitudes."""
+    if case == "grid5x5_sliced":
+        g = json.load(open(golden_dir / "grid5x5_d12.json"))
+        cfg = list(g["batch_base"])
+        for q in g["open_qubits_1based"]:
+            cfg[q - 1] = "x"
+        tn = grid_net(5, 5, 12, g["seed"], "".join(cfg), np.complex128)
+        plan = PL.reconfigure(tn, PL.sweep_plan(tn, PL.grid_site_order(5, 5, open_last=g["open_qubits_1based"])))
+        sliced = PL.choose_slices(tn, plan, 6)[0]
+        kw = dict(sliced=sliced, group_bits=3, precision="mixed")
+        want = np.array([complex(a, b) for a, b in g["batch_amplitudes"]])
+        tol = RTOL["mixed"]
+    else:
+        cfg0 = "0110" + "x" * 4 + "1001" + "01" + "x" * 2
+        rng = np.random.default_rng(5)
+        batch = ["".join("x" if c == "x" else str(int(rng.integers(0, 2))) for c in cfg0) for _ in range(4)]
+        tn = grid_net(4, 4, 8, 7, batch[0], np.complex128)
+        plan = PL.reconfigure(tn, PL.sweep_plan(tn, PL.grid_site_order(4, 4, "col")), windows=(8,))
+        kw = dict(precision="mixed", small_elems=16, bitstrings=batch,
+                  fuse={"tile_bits": 10, "min_elems": 64, "op_elems": 256})
+        sv = O.statevector(16, O.grid_circuit_gates(4, 4, 8, 7))
+        open_q = [i for i, ch in enumerate(cfg0) if ch == "x"]
+        want = []
+        for bits in batch:
+            for j in range(2 ** len(open_q)):
+                full = list(bits)
+                for t, q in enumerate(open_q):
+                    full[q] = str((j >> t) & 1)
+                want.append(sv[sum(int(ch) << q for q, ch in enumerate(full))])
+        want = np.array(want)
+        tol = 2e-6
+    ctx = tn.backend.ctx
+    out = {}
+    for ns in (1, 4):
+        ctx.set_plan_streams(ns)
+        try:
+            p = compile_plan(tn, plan, **kw)
+        finally:
+            ctx.set_plan_streams(4)
+        runs = []
+        for use_graph in (False, True, True, True):
+            p.execute(use_graph)
+            runs.append(p.result().copy())
+        p.close()
+        for r in runs[1:]:
+            np.testing.assert_array_equal(r, runs[0])
+        out[ns] = runs[0]
+    np.testing.assert_array_equal(out[1], out[4])
+    got = out[4].reshape(-1) if case == "grid5x5_sliced" else np.concatenate(
+        [out[4][b].reshape(-1, order="F") for b in range(out[4].shape[0])])
+    if case == "grid5x5_sliced":
+        got = out[4].reshape(-1, order="F")
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < tol
