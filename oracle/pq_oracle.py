@@ -17,7 +17,10 @@ Pins (what checks each part; see tests/test_oracle.py):
   * contract_network, single amplitudes ...... brute-force state vector; plan invariance
   * slicing .................................. sum over slices == unsliced
   * decompose_gate ............................ round trip (test/layer3_tests.jl:57-71)
-  * bgreedy tau->0 ............................ plan validity + ledger re-count
+  * bgreedy tau->0 ............................ hand-worked known answers (alpha = 1 and 0: every
+                                                pair energy, the plan, its costs), planted energy
+                                                mistakes rejected, plan validity + ledger re-count
+                                                (tests/test_greedy_parity.py, tests/test_oracle.py)
   * optimal_contraction_cost .................. hand-counted costs of tiny chains / stars
   * gate tensors .............................. closed forms + unitarity
   * decompose_tensor / MPS drivers ............ exact (untruncated) MPS == brute-force
