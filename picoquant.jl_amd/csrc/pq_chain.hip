@@ -161,6 +161,10 @@ __device__ __forceinline__ void stage(lc* tile, const lc* sd, const int* si_, co
   }
 }
 
+// GV / SV: the gather / scatter moves each element pair as one 16-byte
+// access (a.gvec / a.svec); separate instantiations keep the two address
+// forms out of each other's code.
+template <bool GV, bool SV>
 __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
   extern __shared__ lc smem[];
   const int tid = threadIdx.x, lane = tid & 63;
@@ -205,15 +209,37 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
   const int gt = tid & ((1 << a.ilo) - 1), st = tid & ((1 << a.olo) - 1);
   const int gp0 = tab[a.ilp + gt], sp0 = tab[a.olp + st];
   const int32_t gg0 = tab[a.ilg + gt], sg0 = tab[a.olg + st];
+  // wait for these four loads here (vmcnt(0), other counters untouched):
+  // otherwise the compiler, merging the loop's entry and back edge, keeps
+  // them pending and waits for every vector memory operation in flight
+  // before their first use inside the loop
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  // spectator offsets (X, Y) of tile t: the host's per-tile table, or
+  // decoded from the spectator modes (uniform across the workgroup)
+  auto offsets = [&](uint32_t t, int32_t& ox, int32_t& oy) {
+    if (a.tof >= 0) {
+      // uniform address in the constant address space: a scalar load, which
+      // waits on no vector memory operation
+      const __attribute__((address_space(4))) int32_t* ct =
+          (const __attribute__((address_space(4))) int32_t*)(tab + a.tof);
+      ox = ct[2 * t];
+      oy = ct[2 * t + 1];
+    } else {
+      group_offsets(a.spec, t, ox, oy);
+    }
+    ox = __builtin_amdgcn_readfirstlane(ox);
+    oy = __builtin_amdgcn_readfirstlane(oy);
+  };
+  // Element accesses are a uniform tile base plus a 32-bit byte offset per
+  // lane (chained tensors hold fewer than 2^29 elements, checked on the
+  // host), so each access needs one address register.
   c64 P[2 * CH_NIT];   // the next tile, in flight (pairs)
-  auto load = [&](uint32_t t) {
-    int32_t ox, oy;
-    group_offsets(a.spec, t, ox, oy);
-    const c64* src = a.X + ox + gg0;
-    int o[CH_NIT];
+  auto load = [&](int32_t ox) {
+    const char* src = reinterpret_cast<const char*>(a.X + ox);
+    uint32_t o[CH_NIT];
 #pragma unroll
-    for (int it = 0; it < CH_NIT; ++it) o[it] = hid[CH_NIT + it];
-    if (a.gvec) {
+    for (int it = 0; it < CH_NIT; ++it) o[it] = (uint32_t)(gg0 + hid[CH_NIT + it]) * 8u;
+    if constexpr (GV) {
 #pragma unroll
       for (int it = 0; it < CH_NIT; ++it) {
         const float4 v = *reinterpret_cast<const float4*>(src + o[it]);
@@ -223,10 +249,11 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
         P[2 * it + 1].y = v.w;
       }
     } else {
+      const uint32_t ps = (uint32_t)a.gps * 8u;
 #pragma unroll
       for (int it = 0; it < CH_NIT; ++it) {
-        P[2 * it] = src[o[it]];
-        P[2 * it + 1] = src[o[it] + a.gps];
+        P[2 * it] = *reinterpret_cast<const c64*>(src + o[it]);
+        P[2 * it + 1] = *reinterpret_cast<const c64*>(src + o[it] + ps);
       }
     }
   };
@@ -239,8 +266,8 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
     }
   };
   // scatter in two halves: the finished tile is read from LDS into
-  // registers (readq), and its stores are issued later (storeq), while the
-  // next tile's first stage runs
+  // registers (readq), and its stores are issued once the next tile is in
+  // LDS (storeq)
   lc Q[2 * CH_NIT];
   auto readq = [&](const lc* buf) {
 #pragma unroll
@@ -250,14 +277,12 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
       Q[2 * it + 1] = buf[q ^ a.spa];
     }
   };
-  auto storeq = [&](uint32_t t) {
-    int32_t ox, oy;
-    group_offsets(a.spec, t, ox, oy);
-    c64* dst = a.Y + oy + sg0;
-    int o[CH_NIT];
+  auto storeq = [&](int32_t oy) {
+    char* dst = reinterpret_cast<char*>(a.Y + oy);
+    uint32_t o[CH_NIT];
 #pragma unroll
-    for (int it = 0; it < CH_NIT; ++it) o[it] = hid[3 * CH_NIT + it];
-    if (a.svec) {
+    for (int it = 0; it < CH_NIT; ++it) o[it] = (uint32_t)(sg0 + hid[3 * CH_NIT + it]) * 8u;
+    if constexpr (SV) {
 #pragma unroll
       for (int it = 0; it < CH_NIT; ++it) {
         float4 w;
@@ -268,6 +293,7 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
         *reinterpret_cast<float4*>(dst + o[it]) = w;
       }
     } else {
+      const uint32_t ps = (uint32_t)a.sps * 8u;
 #pragma unroll
       for (int it = 0; it < CH_NIT; ++it) {
         c64 w0, w1;
@@ -275,8 +301,8 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
         w0.y = Q[2 * it].y;
         w1.x = Q[2 * it + 1].x;
         w1.y = Q[2 * it + 1].y;
-        dst[o[it]] = w0;
-        dst[o[it] + a.sps] = w1;
+        *reinterpret_cast<c64*>(dst + o[it]) = w0;
+        *reinterpret_cast<c64*>(dst + o[it] + ps) = w1;
       }
     }
   };
@@ -288,22 +314,31 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
   // again) issues at a higher wave priority, so its stages run ahead and
   // its memory phase overlaps the other's stages (about 2 % per chain).
   if (blockIdx.x < G / 2) __builtin_amdgcn_s_setprio(3);
+  int32_t cx = 0, cy = 0, nx = 0, ny = 0;   // offsets of the current and the next tile
   if (t < total) {
-    load(t);
+    offsets(t, cx, cy);
+    load(cx);
     put(tile);
+    if (t + G < total) offsets(t + G, nx, ny);
   }
   __syncthreads();
 
   // every wave of a workgroup runs the same iterations, and the loop ends
-  // when the workgroup's tiles run out.  Per tile: the previous tile's
-  // stores are issued, then the stages run (the next tile's loads are
-  // issued after the first: the memory pipeline is otherwise idle during
-  // the stages); the finished tile is read into registers and the
-  // prefetched one put into LDS.
-  bool held = false;    // Q holds tile tq, not yet stored
-  uint32_t tq = 0;
+  // when the workgroup's tiles run out.  Per tile: the next tile's loads are
+  // issued (into registers; the memory pipeline is otherwise idle during the
+  // stages), the stages run, the finished tile is read into registers, the
+  // prefetched one is put into LDS, and the finished tile's stores are
+  // issued; they drain during the next tile's stages.  Stores follow the
+  // loads they could delay: loads and stores share the vector memory
+  // counter, which retires in order, so waiting for a load never waits for a
+  // store issued after it.
   for (; t < total; t += G) {
-    if (held) storeq(tq);
+    const bool more = t + G < total;
+    int32_t mx = 0, my = 0;
+    if (more) {
+      load(nx);
+      if (t + 2 * G < total) offsets(t + 2 * G, mx, my);
+    }
     for (int s = 0; s < a.nst; ++s) {
       const ChainStage& g = a.st[s];
       const int nkc = ((1 << g.nk) + 3) / 4, nmb = ((1 << g.nm) + 15) / 16;
@@ -317,17 +352,17 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
         case 4 * 4 + 2: stage<4, 2>(tile, sd, sidx, g, wave, li, lg); break;
         default: stage<8, 2>(tile, sd, sidx, g, wave, li, lg); break;
       }
-      if (s == 0 && t + G < total) load(t + G);
       __syncthreads();
     }
     readq(tile);
-    held = true;
-    tq = t;
     __syncthreads();
-    if (t + G < total) put(tile);
+    if (more) put(tile);
+    storeq(cy);
+    cy = ny;
+    nx = mx;
+    ny = my;
     __syncthreads();
   }
-  if (held) storeq(tq);
 }
 
 }  // namespace
@@ -354,10 +389,17 @@ hipError_t launch_chain(const ChainArgs& a, hipStream_t s) {
     if (ncu[dev] < 1) ncu[dev] = 1;
   }
   int per_cu = 0;
-  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_chain, CH_THREADS, lds);
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_chain<true, true>, CH_THREADS, lds);
   if (e != hipSuccess) return e;
   per_cu = std::max(per_cu, 1);
   const uint32_t grid = std::min<uint32_t>(a.spec.total, (uint32_t)(per_cu * ncu[dev]));
-  hipLaunchKernelGGL(k_chain, dim3(grid), dim3(CH_THREADS), lds, s, a);
+  if (a.gvec && a.svec)
+    hipLaunchKernelGGL((k_chain<true, true>), dim3(grid), dim3(CH_THREADS), lds, s, a);
+  else if (a.gvec)
+    hipLaunchKernelGGL((k_chain<true, false>), dim3(grid), dim3(CH_THREADS), lds, s, a);
+  else if (a.svec)
+    hipLaunchKernelGGL((k_chain<false, true>), dim3(grid), dim3(CH_THREADS), lds, s, a);
+  else
+    hipLaunchKernelGGL((k_chain<false, false>), dim3(grid), dim3(CH_THREADS), lds, s, a);
   return hipGetLastError();
 }

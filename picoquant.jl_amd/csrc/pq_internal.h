@@ -143,8 +143,10 @@ struct ChainArgs {
   int32_t gpa, gps, spa, sps;
   int32_t gvec_ok, svec_ok;                // strides allow it (host); the launch also checks alignment
   int32_t gvec, svec;
+  int32_t tof;  // table offset of the per-tile (X, Y) spectator offsets, -1: decode spec per tile
   Group spec;   // spectator modes: s0 = stride in X, s1 = stride in Y
   ChainStage st[CH_ST];
 };
 #define CH_STAGE_BYTES PQ_CHAIN_STAGE_BYTES   // LDS after the tile for the staged operators and offsets
+#define CH_TOF_TILES (1 << 21)   // chains with at most this many tiles get a per-tile offset table
 hipError_t launch_chain(const ChainArgs& a, hipStream_t s);

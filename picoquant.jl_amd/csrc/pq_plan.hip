@@ -612,6 +612,25 @@ const char* parse_chain(pq_ctx* c, pq_plan* p, const pq_plan_desc* d, int i, PSt
     g.rl = bit_table(tab, rps[t], 0, g.rlo, true, lay);
     g.rh = bit_table(tab, rps[t], g.rlo, nr, true, lay);
   }
+  // per-tile spectator offsets (X, Y), decoded here once instead of per tile
+  // on the device; tile t's pair at tab[tof + 2 t]
+  a.tof = -1;
+  if (a.spec.total <= CH_TOF_TILES) {
+    a.tof = (int32_t)tab.size();
+    std::vector<int64_t> ext(spec.size());
+    for (size_t m = 0; m < spec.size(); ++m) ext[m] = spec[m].ext;
+    for (int64_t t = 0; t < (int64_t)a.spec.total; ++t) {
+      int64_t r = t, ox = 0, oy = 0;
+      for (size_t m = 0; m < spec.size(); ++m) {
+        const int64_t v = r % ext[m];
+        r /= ext[m];
+        ox += v * spec[m].s0;
+        oy += v * spec[m].s1;
+      }
+      tab.push_back((int32_t)ox);
+      tab.push_back((int32_t)oy);
+    }
+  }
   st.bytes += 8.0 * (double)a.spec.total * (double)(((int64_t)1 << nin) + ((int64_t)1 << nout));
   st.spec.kc = PQ_KC_CHAIN;
   st.spec.tile = -1;
