@@ -42,18 +42,27 @@ struct __attribute__((aligned(8))) lc {
 // of every access below fall on distinct banks; addresses of disjoint bit
 // sets combine by XOR.
 
+// Tile addresses are byte offsets (the host's element addresses times 8,
+// scaled when staged): an access is the XOR of table entries, with no shift.
+__device__ __forceinline__ lc& at(lc* t, int b) { return *reinterpret_cast<lc*>(reinterpret_cast<char*>(t) + b); }
+__device__ __forceinline__ const lc& at(const lc* t, int b) {
+  return *reinterpret_cast<const lc*>(reinterpret_cast<const char*>(t) + b);
+}
+
 __device__ __forceinline__ floatx4 mfma(float a, float b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
 // One stage, T[r, m] = sum_k S[m, k] T[r, k], with NKC chunks of 4 K values
 // and NMB blocks of 16 M values (K = 2^nk <= 4 NKC, M = 2^nm <= 16 NMB).
+// FULL: every lane writes all its outputs (16 low R values and M = 16 NMB),
+// so the writes need no per-lane predicate.
 // Wave w of the CH_THREADS / 64 takes the blocks of 16 r's numbered w,
 // w + CH_THREADS / 64, ...  The lane's operator values (MFMA A operand:
 // row m = 16 mb + li, column k = 4 kc + lg) stay in registers when there
 // are at most 8 chunk-blocks of them; the 32 x 32 case rereads them from
 // LDS per M block (register room for the next tile's prefetch).
-template <int NKC, int NMB>
+template <int NKC, int NMB, bool FULL>
 __device__ __forceinline__ void stage(lc* tile, const lc* sd, const int* si_, const ChainStage& g, int wave, int li,
                                       int lg) {
   constexpr bool HOLD = NKC * NMB <= 8;
@@ -88,13 +97,11 @@ __device__ __forceinline__ void stage(lc* tile, const lc* sd, const int* si_, co
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int m = 16 * mb + 4 * lg + i;
-      mof[mb][i] = m < nm ? si_[g.mo + m] : -1;
+      mof[mb][i] = m < nm ? si_[g.mo + m] : 0;
     }
   const bool rv = li < (1 << g.rlo);
   const int rlo = si_[g.ro + (rv ? li : 0)];
   const bool kpart = NKC == 1 && nk < 4;   // fewer K values than lane groups
-  // every lane stores all its outputs (no partial R or M block)
-  const bool full = (1 << g.rlo) == 16 && nm >= 16 * NMB;
   const int nrb = 1 << g.rhi;
   constexpr int NW = CH_THREADS / 64;
   // UNR blocks of 16 r's per iteration: the K values of all of them are
@@ -112,7 +119,7 @@ __device__ __forceinline__ void stage(lc* tile, const lc* sd, const int* si_, co
       roff[u] = rlo ^ si_[g.rho + rb];
 #pragma unroll
       for (int kc = 0; kc < NKC; ++kc) {
-        lc v = tile[roff[u] ^ kof[kc]];
+        lc v = at(tile, roff[u] ^ kof[kc]);
         if (kpart && lg >= nk) {
           v.x = 0.f;
           v.y = 0.f;
@@ -148,11 +155,12 @@ __device__ __forceinline__ void stage(lc* tile, const lc* sd, const int* si_, co
         if (wr) {
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            if (full || (rv && mof[mb][i] >= 0)) {
-              lc v;
-              v.x = cr[i];
-              v.y = ci[i];
-              tile[roff[u] ^ mof[mb][i]] = v;
+            if (FULL || (rv && 16 * mb + 4 * lg + i < nm)) {
+              // real and imaginary parts written from the two accumulators
+              // as they are (one ds_write2_b32), no register pairing
+              float* e = &at(tile, roff[u] ^ mof[mb][i]).x;
+              e[0] = cr[i];
+              e[1] = ci[i];
             }
           }
         }
@@ -187,10 +195,10 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
       const c64 v = g.S[tab[g.sk + k] + tab[g.sm + m]];
       sd[g.sd + e] = lc{v.x, v.y};
     }
-    for (int e = tid; e < nk; e += CH_THREADS) sidx[g.ko + e] = tab[g.kp + e];
-    for (int e = tid; e < (1 << g.nm); e += CH_THREADS) sidx[g.mo + e] = tab[g.mp + e];
-    for (int e = tid; e < (1 << g.rlo); e += CH_THREADS) sidx[g.ro + e] = tab[g.rl + e];
-    for (int e = tid; e < (1 << g.rhi); e += CH_THREADS) sidx[g.rho + e] = tab[g.rh + e];
+    for (int e = tid; e < nk; e += CH_THREADS) sidx[g.ko + e] = 8 * tab[g.kp + e];
+    for (int e = tid; e < (1 << g.nm); e += CH_THREADS) sidx[g.mo + e] = 8 * tab[g.mp + e];
+    for (int e = tid; e < (1 << g.rlo); e += CH_THREADS) sidx[g.ro + e] = 8 * tab[g.rl + e];
+    for (int e = tid; e < (1 << g.rhi); e += CH_THREADS) sidx[g.rho + e] = 8 * tab[g.rh + e];
   }
   // Gather / scatter without branches: lane l, iteration it moves the
   // element pair (l + 2^lo it) along the side's pair bit; each lane does
@@ -200,14 +208,15 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
   // positions, or stores the same values to the same addresses.
   int* hid = sidx + a.idn;   // tile address / tensor offset per iteration
   if (tid < CH_NIT) {
-    hid[tid] = tab[a.ihp + (tid & ((1 << a.ihi) - 1))];
+    hid[tid] = 8 * tab[a.ihp + (tid & ((1 << a.ihi) - 1))];
     hid[CH_NIT + tid] = tab[a.ihg + (tid & ((1 << a.ihi) - 1))];
-    hid[2 * CH_NIT + tid] = tab[a.ohp + (tid & ((1 << a.ohi) - 1))];
+    hid[2 * CH_NIT + tid] = 8 * tab[a.ohp + (tid & ((1 << a.ohi) - 1))];
     hid[3 * CH_NIT + tid] = tab[a.ohg + (tid & ((1 << a.ohi) - 1))];
   }
   __syncthreads();
   const int gt = tid & ((1 << a.ilo) - 1), st = tid & ((1 << a.olo) - 1);
-  const int gp0 = tab[a.ilp + gt], sp0 = tab[a.olp + st];
+  const int gp0 = 8 * tab[a.ilp + gt], sp0 = 8 * tab[a.olp + st];
+  const int gpa = 8 * a.gpa, spa = 8 * a.spa;
   const int32_t gg0 = tab[a.ilg + gt], sg0 = tab[a.olg + st];
   // wait for these four loads here (vmcnt(0), other counters untouched):
   // otherwise the compiler, merging the loop's entry and back edge, keeps
@@ -261,8 +270,8 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
 #pragma unroll
     for (int it = 0; it < CH_NIT; ++it) {
       const int q = gp0 ^ hid[it];
-      buf[q] = lc{P[2 * it].x, P[2 * it].y};
-      buf[q ^ a.gpa] = lc{P[2 * it + 1].x, P[2 * it + 1].y};
+      at(buf, q) = lc{P[2 * it].x, P[2 * it].y};
+      at(buf, q ^ gpa) = lc{P[2 * it + 1].x, P[2 * it + 1].y};
     }
   };
   // scatter in two halves: the finished tile is read from LDS into
@@ -273,8 +282,8 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
 #pragma unroll
     for (int it = 0; it < CH_NIT; ++it) {
       const int q = sp0 ^ hid[2 * CH_NIT + it];
-      Q[2 * it] = buf[q];
-      Q[2 * it + 1] = buf[q ^ a.spa];
+      Q[2 * it] = at(buf, q);
+      Q[2 * it + 1] = at(buf, q ^ spa);
     }
   };
   auto storeq = [&](int32_t oy) {
@@ -342,15 +351,24 @@ __global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
     for (int s = 0; s < a.nst; ++s) {
       const ChainStage& g = a.st[s];
       const int nkc = ((1 << g.nk) + 3) / 4, nmb = ((1 << g.nm) + 15) / 16;
-      switch (nkc * 4 + nmb) {
-        case 1 * 4 + 1: stage<1, 1>(tile, sd, sidx, g, wave, li, lg); break;
-        case 2 * 4 + 1: stage<2, 1>(tile, sd, sidx, g, wave, li, lg); break;
-        case 4 * 4 + 1: stage<4, 1>(tile, sd, sidx, g, wave, li, lg); break;
-        case 8 * 4 + 1: stage<8, 1>(tile, sd, sidx, g, wave, li, lg); break;
-        case 1 * 4 + 2: stage<1, 2>(tile, sd, sidx, g, wave, li, lg); break;
-        case 2 * 4 + 2: stage<2, 2>(tile, sd, sidx, g, wave, li, lg); break;
-        case 4 * 4 + 2: stage<4, 2>(tile, sd, sidx, g, wave, li, lg); break;
-        default: stage<8, 2>(tile, sd, sidx, g, wave, li, lg); break;
+      // FULL: 16 low R values (rlo = 4) and every M row of the blocks used
+      const bool full = g.rlo == 4 && g.nm >= 4 + (nmb == 2);
+      switch ((nkc * 4 + nmb) * 2 + (full ? 1 : 0)) {
+        case (4 * 4 + 1) * 2 + 1: stage<4, 1, true>(tile, sd, sidx, g, wave, li, lg); break;
+        case (2 * 4 + 2) * 2 + 1: stage<2, 2, true>(tile, sd, sidx, g, wave, li, lg); break;
+        case (1 * 4 + 1) * 2:
+        case (1 * 4 + 1) * 2 + 1: stage<1, 1, false>(tile, sd, sidx, g, wave, li, lg); break;
+        case (2 * 4 + 1) * 2:
+        case (2 * 4 + 1) * 2 + 1: stage<2, 1, false>(tile, sd, sidx, g, wave, li, lg); break;
+        case (4 * 4 + 1) * 2: stage<4, 1, false>(tile, sd, sidx, g, wave, li, lg); break;
+        case (8 * 4 + 1) * 2:
+        case (8 * 4 + 1) * 2 + 1: stage<8, 1, false>(tile, sd, sidx, g, wave, li, lg); break;
+        case (1 * 4 + 2) * 2:
+        case (1 * 4 + 2) * 2 + 1: stage<1, 2, false>(tile, sd, sidx, g, wave, li, lg); break;
+        case (2 * 4 + 2) * 2: stage<2, 2, false>(tile, sd, sidx, g, wave, li, lg); break;
+        case (4 * 4 + 2) * 2:
+        case (4 * 4 + 2) * 2 + 1: stage<4, 2, false>(tile, sd, sidx, g, wave, li, lg); break;
+        default: stage<8, 2, false>(tile, sd, sidx, g, wave, li, lg); break;
       }
       __syncthreads();
     }
