@@ -68,8 +68,10 @@ hipError_t launch_multi(const MultiDesc* descs, int n, unsigned blocks_x, unsign
 // Every output element of a step is one item (step, A offset, B offset,
 // C offset), precomputed on the host; a step's K offsets (A, B) are
 // ktab[kt .. kt + nk).  Conversions: item = (step, source offset, -, target
-// offset).  lv[b] is the offset in lv of tree b's entry: nlevels, then the
-// nlevels + 1 item boundaries of its levels.
+// offset).  lv[b] is the offset in lv of tree b's entry: nlevels, its first
+// step record and number of step records, its first K table entry and
+// number of K entries (a tree's records and K entries are contiguous), then
+// the nlevels + 1 item boundaries of its levels.
 struct ForestStep {
   const void* A;
   const void* B;
@@ -78,8 +80,11 @@ struct ForestStep {
   int32_t conv;       // 1: element-type conversion A -> C (the other type)
   int32_t c128;       // computes in (converts from) complex128
 };
+// lds_bytes: LDS for the largest tree's step records and K offsets (0: the
+// kernel reads them from global memory), at most PQ_FOREST_LDS_BYTES
+#define PQ_FOREST_LDS_BYTES (48 << 10)
 hipError_t launch_forest(const ForestStep* steps, const int4* items, const int2* ktab, const int32_t* lv,
-                         int ntrees, hipStream_t s);
+                         int ntrees, size_t lds_bytes, hipStream_t s);
 
 template <typename C>
 hipError_t launch_stream(const C* X, const C* Y, C* Out, const StreamArgs& a, hipStream_t s);

@@ -250,48 +250,75 @@ __global__ __launch_bounds__(256) void k_multi(const MultiDesc* __restrict__ des
 // ------------------------------------------------------------------ k_forest
 // One item: an output element of a small unbatched step (one lane, the K
 // offsets from the step's table), or one element of a conversion.
+// Tensor pointers of the step records are global memory (the records may
+// sit in LDS, so the compiler is told the address space).
+#define PQ_GLOBAL __attribute__((address_space(1)))
 template <typename C>
-__device__ __forceinline__ void forest_item(const ForestStep& f, const int4& it, const int2* __restrict__ ktab) {
+__device__ __forceinline__ void forest_item(const ForestStep& f, const int4& it, const int2* ktab) {
   if (f.conv) {
     typedef typename std::conditional<sizeof(C) == 16, c64, c128>::type D;
-    const C v = reinterpret_cast<const C*>(f.A)[it.y];
+    const C v = ((const PQ_GLOBAL C*)f.A)[it.y];
     D w;
     w.x = (decltype(w.x))v.x;
     w.y = (decltype(w.y))v.y;
-    reinterpret_cast<D*>(f.C)[it.w] = w;
+    ((PQ_GLOBAL D*)f.C)[it.w] = w;
     return;
   }
-  const C* A = reinterpret_cast<const C*>(f.A) + it.y;
-  const C* B = reinterpret_cast<const C*>(f.B) + it.z;
+  const PQ_GLOBAL C* A = ((const PQ_GLOBAL C*)f.A) + it.y;
+  const PQ_GLOBAL C* B = ((const PQ_GLOBAL C*)f.B) + it.z;
   C acc = czero<C>();
   // the loads of several K values in flight at once (a tree's level waits
   // for its slowest lane)
 #pragma unroll 4
   for (int32_t k = 0; k < f.nk; ++k) {
-    const int2 o = ktab[f.kt + k];
-    cfma(acc, A[o.x], B[o.y]);
+    const int2 o = ktab[k];
+    const C a = A[o.x], b = B[o.y];
+    cfma(acc, a, b);
   }
-  reinterpret_cast<C*>(f.C)[it.w] = acc;
+  ((PQ_GLOBAL C*)f.C)[it.w] = acc;
 }
 
 // Workgroup b runs tree b level by level: a level's items (the outputs of
 // all its steps, which read only earlier levels) are spread over the lanes,
 // and the barrier after each level makes its stores visible to the
-// workgroup's later levels (one workgroup runs on one CU).
+// workgroup's later levels (one workgroup runs on one CU).  Tree header at
+// lv[lv[b]]: levels, first step record, step records, first K entry, K
+// entries, then the levels' item bounds.  LDS: the tree's step records and K
+// offsets are copied into LDS once, so a level waits on global memory only
+// for its operands (the items of the next level are read ahead, as they
+// depend on nothing the level computes).
+template <bool LDS>
 __global__ __launch_bounds__(256) void k_forest(const ForestStep* __restrict__ fs, const int4* __restrict__ items,
                                                 const int2* __restrict__ ktab, const int32_t* __restrict__ lv) {
+  extern __shared__ int32_t fsm[];
   const int32_t q = lv[blockIdx.x];
-  const int32_t nlev = lv[q];
-  const int32_t* bnd = lv + q + 1;
+  const int32_t nlev = lv[q], fs0 = lv[q + 1], nfs = lv[q + 2], k0 = lv[q + 3], nkt = lv[q + 4];
+  const int32_t* bnd = lv + q + 5;
+  const ForestStep* F = fs + fs0;
+  const int2* KT = ktab + k0;
+  if constexpr (LDS) {
+    constexpr int W = (int)(sizeof(ForestStep) / 4);
+    const int32_t* src = reinterpret_cast<const int32_t*>(fs + fs0);
+    for (int32_t e = (int32_t)threadIdx.x; e < nfs * W; e += (int32_t)blockDim.x) fsm[e] = src[e];
+    int2* kd = reinterpret_cast<int2*>(fsm + nfs * W + (nfs * W & 1));
+    for (int32_t e = (int32_t)threadIdx.x; e < nkt; e += (int32_t)blockDim.x) kd[e] = ktab[k0 + e];
+    __syncthreads();
+    F = reinterpret_cast<const ForestStep*>(fsm);
+    KT = kd;
+  }
+  int4 nx = make_int4(0, 0, 0, 0);
+  if (nlev > 0 && bnd[0] + (int32_t)threadIdx.x < bnd[1]) nx = items[bnd[0] + threadIdx.x];
   for (int32_t l = 0; l < nlev; ++l) {
-    const int32_t i1 = bnd[l + 1];
-    for (int32_t i = bnd[l] + (int32_t)threadIdx.x; i < i1; i += (int32_t)blockDim.x) {
-      const int4 it = items[i];
-      const ForestStep& f = fs[it.x];
+    const int32_t i0 = bnd[l] + (int32_t)threadIdx.x, i1 = bnd[l + 1];
+    const int4 cur = nx;
+    if (l + 1 < nlev && i1 + (int32_t)threadIdx.x < bnd[l + 2]) nx = items[i1 + threadIdx.x];
+    for (int32_t i = i0; i < i1; i += (int32_t)blockDim.x) {
+      const int4 it = i == i0 ? cur : items[i];
+      const ForestStep& f = F[it.x - fs0];
       if (f.c128)
-        forest_item<c128>(f, it, ktab);
+        forest_item<c128>(f, it, KT + (f.kt - k0));
       else
-        forest_item<c64>(f, it, ktab);
+        forest_item<c64>(f, it, KT + (f.kt - k0));
     }
     __syncthreads();
   }
@@ -364,9 +391,12 @@ hipError_t launch_multi(const MultiDesc* descs, int n, unsigned blocks_x, unsign
 }
 
 hipError_t launch_forest(const ForestStep* steps, const int4* items, const int2* ktab, const int32_t* lv,
-                         int ntrees, hipStream_t s) {
-  if (ntrees < 1 || ntrees > (1 << 24)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_forest, dim3(ntrees), dim3(256), 0, s, steps, items, ktab, lv);
+                         int ntrees, size_t lds_bytes, hipStream_t s) {
+  if (ntrees < 1 || ntrees > (1 << 24) || lds_bytes > PQ_FOREST_LDS_BYTES) return hipErrorInvalidValue;
+  if (lds_bytes > 0)
+    hipLaunchKernelGGL(k_forest<true>, dim3(ntrees), dim3(256), lds_bytes, s, steps, items, ktab, lv);
+  else
+    hipLaunchKernelGGL(k_forest<false>, dim3(ntrees), dim3(256), 0, s, steps, items, ktab, lv);
   return hipGetLastError();
 }
 

@@ -169,6 +169,7 @@ struct pq_plan {
   const int4* f_items = nullptr;
   const int2* f_ktab = nullptr;
   const int32_t* f_lv = nullptr;
+  size_t f_lds = 0;                  // forest launch: LDS bytes of its tree tables (0: none)
   int n_sliced = 0, group_bits = 0;
   int64_t n_groups = 1;
   int n_out_bits = 0;                // the top n_out_bits sliced bonds number output batches
@@ -707,7 +708,8 @@ int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
     {
       ProfScope ps(c, PQ_KC_BATCHED, by, fl);
       if (L.forest)
-        e = launch_forest(p->f_steps, p->f_items, p->f_ktab, p->f_lv, (int)L.tree_start.size() - 1, c->stream);
+        e = launch_forest(p->f_steps, p->f_items, p->f_ktab, p->f_lv, (int)L.tree_start.size() - 1, p->f_lds,
+                          c->stream);
       else
         e = (L.dtype == PQ_C128)
                 ? launch_multi<c128>(p->descs + L.desc_off, (int)L.steps.size(), L.blocks_x, L.batch, group,
@@ -1267,12 +1269,16 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     std::vector<int2> ktab;
     const int nt = (int)L.tree_start.size() - 1;
     std::vector<int32_t> lv(nt, 0);
+    size_t f_lds = 0;
     for (int b = 0; b < nt; ++b) {
       lv[b] = (int32_t)lv.size();
       std::map<int, std::vector<int>> lvl;   // level -> steps
       for (int q = L.tree_start[b]; q < L.tree_start[b + 1]; ++q)
         lvl[p->steps[L.steps[q]].level].push_back(L.steps[q]);
       lv.push_back((int32_t)lvl.size());
+      const size_t hdr = lv.size();
+      const int32_t fs0 = (int32_t)fsv.size(), k0 = (int32_t)ktab.size();
+      lv.insert(lv.end(), 4, 0);   // first step record, records, first K entry, K entries
       lv.push_back((int32_t)items.size());
       for (auto& kv : lvl) {
         for (int i : kv.second) {
@@ -1307,6 +1313,15 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
         }
         lv.push_back((int32_t)items.size());
       }
+      const int32_t nfs = (int32_t)fsv.size() - fs0, nkt = (int32_t)ktab.size() - k0;
+      lv[hdr] = fs0;
+      lv[hdr + 1] = nfs;
+      lv[hdr + 2] = k0;
+      lv[hdr + 3] = nkt;
+      // LDS of the tree's records and K offsets (k_forest: records as 32-bit
+      // words, then the K entries 8-byte aligned)
+      const size_t words = (size_t)nfs * (sizeof(ForestStep) / 4);
+      f_lds = std::max(f_lds, 4 * (words + (words & 1)) + sizeof(int2) * (size_t)nkt);
     }
     if (items.size() >= ((size_t)1 << 31)) {
       free_plan(c, p);
@@ -1333,6 +1348,8 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     p->f_items = (const int4*)(fb + o_items);
     p->f_ktab = (const int2*)(fb + o_k);
     p->f_lv = (const int32_t*)(fb + o_lv);
+    // the tables in LDS when the largest tree's fit, else read from memory
+    p->f_lds = f_lds <= PQ_FOREST_LDS_BYTES ? f_lds : 0;
   }
   if (!chain_tab.empty()) {
     if (hipMalloc((void**)&p->chain_tab, chain_tab.size() * sizeof(int32_t)) != hipSuccess ||
