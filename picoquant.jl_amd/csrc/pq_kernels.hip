@@ -250,32 +250,28 @@ __global__ __launch_bounds__(256) void k_multi(const MultiDesc* __restrict__ des
 // ------------------------------------------------------------------ k_forest
 // One item: an output element of a small unbatched step (one lane, the K
 // offsets from the step's table), or one element of a conversion.
-// Tensor pointers of the step records are global memory (the records may
-// sit in LDS, so the compiler is told the address space).
-#define PQ_GLOBAL __attribute__((address_space(1)))
 template <typename C>
 __device__ __forceinline__ void forest_item(const ForestStep& f, const int4& it, const int2* ktab) {
   if (f.conv) {
     typedef typename std::conditional<sizeof(C) == 16, c64, c128>::type D;
-    const C v = ((const PQ_GLOBAL C*)f.A)[it.y];
+    const C v = reinterpret_cast<const C*>(f.A)[it.y];
     D w;
     w.x = (decltype(w.x))v.x;
     w.y = (decltype(w.y))v.y;
-    ((PQ_GLOBAL D*)f.C)[it.w] = w;
+    reinterpret_cast<D*>(f.C)[it.w] = w;
     return;
   }
-  const PQ_GLOBAL C* A = ((const PQ_GLOBAL C*)f.A) + it.y;
-  const PQ_GLOBAL C* B = ((const PQ_GLOBAL C*)f.B) + it.z;
+  const C* A = reinterpret_cast<const C*>(f.A) + it.y;
+  const C* B = reinterpret_cast<const C*>(f.B) + it.z;
   C acc = czero<C>();
   // the loads of several K values in flight at once (a tree's level waits
   // for its slowest lane)
 #pragma unroll 4
   for (int32_t k = 0; k < f.nk; ++k) {
     const int2 o = ktab[k];
-    const C a = A[o.x], b = B[o.y];
-    cfma(acc, a, b);
+    cfma(acc, A[o.x], B[o.y]);
   }
-  ((PQ_GLOBAL C*)f.C)[it.w] = acc;
+  reinterpret_cast<C*>(f.C)[it.w] = acc;
 }
 
 // Workgroup b runs tree b level by level: a level's items (the outputs of
