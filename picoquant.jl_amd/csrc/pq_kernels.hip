@@ -23,11 +23,14 @@ template <typename C>
 __device__ __forceinline__ C ldg(const C* p) { return *p; }
 
 // ---------------------------------------------------------------- k_stream
+// The lane's first X-open index has its K values loaded before the small
+// operand is staged in LDS: those loads depend on nothing the staging writes,
+// so the kernel waits for global memory once before the barrier, not twice.
 template <typename C, int KP>
 __global__ __launch_bounds__(256) void k_stream(const C* __restrict__ X, const C* __restrict__ Y,
                                                 C* __restrict__ Out, StreamArgs a) {
   extern __shared__ unsigned char smem_raw[];
-  const uint32_t Kt = a.K.total, Nt = a.N.total;
+  const uint32_t Kt = a.K.total, Nt = a.N.total, Mt = a.M.total;
   {
     int64_t ox, oy, oc;
     batch_offsets(a.L, blockIdx.z, ox, oy, oc);
@@ -35,9 +38,25 @@ __global__ __launch_bounds__(256) void k_stream(const C* __restrict__ X, const C
     Y += oy;
     Out += oc;
   }
+  // K offsets in X (uniform)
+  int32_t xk[KP];
+#pragma unroll
+  for (int k = 0; k < KP; ++k) {
+    int32_t ox = 0, oy = 0;
+    if (k < (int)Kt) group_offsets(a.K, (uint32_t)k, ox, oy);
+    xk[k] = ox;
+  }
+  uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  C xv[KP];
+  int32_t oc = 0;
+  if (m < Mt) {
+    int32_t ox;
+    group_offsets(a.M, m, ox, oc);
+#pragma unroll
+    for (int k = 0; k < KP; ++k) xv[k] = (k < (int)Kt) ? ldg(X + ox + xk[k]) : czero<C>();
+  }
   C* ys = reinterpret_cast<C*>(smem_raw);                              // [KP][Nt]
-  int32_t* xk = reinterpret_cast<int32_t*>(ys + (size_t)KP * Nt);      // [KP]
-  int32_t* cn = xk + KP;                                               // [Nt]
+  int32_t* cn = reinterpret_cast<int32_t*>(ys + (size_t)KP * Nt);      // [Nt]
   for (uint32_t t = threadIdx.x; t < KP * Nt; t += blockDim.x) {
     uint32_t k = t % KP, n = t / KP;
     C v = czero<C>();
@@ -49,24 +68,19 @@ __global__ __launch_bounds__(256) void k_stream(const C* __restrict__ X, const C
     }
     ys[(size_t)n * KP + k] = v;
   }
-  for (uint32_t t = threadIdx.x; t < KP; t += blockDim.x) {
-    int32_t ox = 0, oy = 0;
-    if (t < Kt) group_offsets(a.K, t, ox, oy);
-    xk[t] = ox;
-  }
   for (uint32_t t = threadIdx.x; t < Nt; t += blockDim.x) {
-    int32_t oy, oc;
-    group_offsets(a.N, t, oy, oc);
-    cn[t] = oc;
+    int32_t oy, ocn;
+    group_offsets(a.N, t, oy, ocn);
+    cn[t] = ocn;
   }
   __syncthreads();
-  const uint32_t Mt = a.M.total;
-  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < Mt; m += gridDim.x * blockDim.x) {
-    int32_t ox, oc;
-    group_offsets(a.M, m, ox, oc);
-    C xv[KP];
+  for (bool first = true; m < Mt; m += gridDim.x * blockDim.x, first = false) {
+    if (!first) {
+      int32_t ox;
+      group_offsets(a.M, m, ox, oc);
 #pragma unroll
-    for (int k = 0; k < KP; ++k) xv[k] = (k < (int)Kt) ? ldg(X + ox + xk[k]) : czero<C>();
+      for (int k = 0; k < KP; ++k) xv[k] = (k < (int)Kt) ? ldg(X + ox + xk[k]) : czero<C>();
+    }
     for (uint32_t n = 0; n < Nt; ++n) {
       C acc = czero<C>();
       const C* yc = ys + (size_t)n * KP;
@@ -334,7 +348,7 @@ hipError_t launch_stream(const C* X, const C* Y, C* Out, const StreamArgs& a, hi
   const uint32_t Kt = a.K.total, Nt = a.N.total;
   int KP = 1;
   while ((uint32_t)KP < Kt) KP <<= 1;
-  size_t smem = (size_t)KP * Nt * sizeof(C) + (size_t)KP * 4 + (size_t)Nt * 4;
+  size_t smem = (size_t)KP * Nt * sizeof(C) + (size_t)Nt * 4;
   unsigned grid = grid_for(a.M.total, 256, a.L.total > 1 ? 1024 : 4096);
   switch (KP) {
 #define PQ_CASE(P) \
