@@ -231,6 +231,13 @@ typedef struct {
                                        (bond n_sliced - n_out_bits + k = bit k of b).  A batch of
                                        bitstrings: each closed output qubit is a leaf with one
                                        column per bitstring, read through these bonds' strides */
+  int32_t n_out_qubits;             /* closed output qubits of such a batch (0: none), for
+                                       pq_plan_set_bitstrings */
+  const int32_t* out_qubit_slot;    /* [n_out_qubits] the qubit's leaf slot: a contiguous
+                                       [2, 2^n_out_bits] tensor, column b = the qubit's output
+                                       vector for bitstring b ([1, 0] for 0, [0, 1] for 1,
+                                       src/layer3.jl:292-313) */
+  const int32_t* out_qubit_bit;     /* [n_out_qubits] the qubit's bit in a bitstring word (0-63) */
 } pq_plan_desc;
 
 /* A fused chain applies L small complex64 operators one after the other to
@@ -277,6 +284,30 @@ int pq_plan_stats(const pq_plan* plan, int64_t* arena_bytes, int64_t* launches_p
                   double* alg_flops, double* alg_bytes, int64_t* out_elems);
 int pq_plan_step_info(const pq_plan* plan, int step, int* kclass, int* tile, int* dependent,
                       double* alg_flops, double* alg_bytes);
+/* Exact cost of one run: complex multiply-adds and algorithmic bytes (each
+ * operand read once, each result written once) summed over steps and slice
+ * groups, as the reference's Metrics ledger counts MACs
+ * (src/backends.jl:38-55); per-step figures into step_macs / step_bytes when
+ * they are not NULL (pq_plan_stats' step count entries each). */
+typedef struct {
+  double macs;
+  double bytes;
+  int64_t steps;
+  int64_t launches;
+} pq_cost;
+int pq_plan_cost(const pq_plan* plan, pq_cost* total, double* step_macs, double* step_bytes);
+/* A batch of output bitstrings for a plan compiled with n_out_qubits > 0:
+ * bitstring b's word has bit out_qubit_bit[q] = the value closed output
+ * qubit q takes (the reference's add_output! per bitstring,
+ * src/layer3.jl:292-313).  1 <= n_bitstrings <= 2^n_out_bits; a shorter
+ * batch is padded with copies of the first bitstring.  Rewrites the qubits'
+ * leaves in place (stream-ordered before later runs), so a captured graph
+ * computes the new batch on its next replay. */
+int pq_plan_set_bitstrings(pq_ctx* ctx, pq_plan* plan, int n_bitstrings, const uint64_t* bitstrings);
+/* One call for the batched path: the bitstrings (if n_bitstrings > 0), one
+ * graph-replayed run over every slice, and the FP64 result into out_fp64
+ * (if not NULL; as pq_plan_result, one block per bitstring). */
+int pq_plan_run(pq_ctx* ctx, pq_plan* plan, int n_bitstrings, const uint64_t* bitstrings, double* out_fp64);
 /* One eager run with HIP events around every step; step_ms[i] = time of step i
  * summed over all slice groups. */
 int pq_plan_profile(pq_ctx* ctx, pq_plan* plan, double* step_ms);

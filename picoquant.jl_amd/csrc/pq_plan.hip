@@ -172,6 +172,10 @@ struct pq_plan {
   size_t f_lds = 0;                  // forest launch: LDS bytes of its tree tables (0: none)
   int n_sliced = 0, group_bits = 0;
   int64_t n_groups = 1;
+  struct OutQubit {
+    int slot, bit;                   // leaf slot of a closed output qubit, its bit in a bitstring word
+  };
+  std::vector<OutQubit> outq;        // a bitstring batch's closed output qubits (pq_plan_set_bitstrings)
   int n_out_bits = 0;                // the top n_out_bits sliced bonds number output batches
   int64_t n_out = 1;                 // 2^n_out_bits results of out_elems each
   int out_slot = -1;
@@ -850,6 +854,19 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
       return bad("slice strides exceed the slot");
     }
   }
+  // Closed output qubits of a bitstring batch: contiguous [2, 2^n_out_bits]
+  // leaves, one column per bitstring.
+  if (d->n_out_qubits < 0 || (d->n_out_qubits > 0 && (!d->out_qubit_slot || !d->out_qubit_bit)))
+    return bad("bad output-qubit list");
+  for (int q = 0; q < d->n_out_qubits; ++q) {
+    const int s = d->out_qubit_slot[q], b = d->out_qubit_bit[q];
+    if (s < 0 || s >= d->n_slots || !p->slots[s].leaf) return bad("an output qubit's slot is not a leaf");
+    if (b < 0 || b > 63) return bad("output qubit bit out of range");
+    const Tensor* t = get(c, p->slots[s].leaf);
+    if (!is_contiguous(*t) || numel(*t) != ((int64_t)2 << d->n_out_bits))
+      return bad("an output qubit's leaf must be a contiguous [2, 2^n_out_bits] tensor");
+    p->outq.push_back({s, b});
+  }
   // Steps.
   int64_t mpos = 0;
   std::vector<int32_t> chain_tab;   // fused-chain address tables (uploaded below)
@@ -1489,6 +1506,65 @@ int pq_set_plan_streams(pq_ctx* c, int n) {
   if (n > 1) return fail(c, PQ_ERR_UNSUPPORTED, "concurrent plan streams are switched off in this build");
   c->plan_streams = n;
   return PQ_OK;
+}
+
+int pq_plan_cost(const pq_plan* p, pq_cost* total, double* step_macs, double* step_bytes) {
+  if (!p) return PQ_ERR_ARG;
+  double macs = 0, bytes = 0;
+  for (size_t i = 0; i < p->steps.size(); ++i) {
+    const PStep& st = p->steps[i];
+    // algorithmic flops are 8 real flops per complex multiply-add; a
+    // conversion moves data only
+    const double m = st.kind == 1 ? 0.0 : st.flops / 8.0;
+    if (step_macs) step_macs[i] = m;
+    if (step_bytes) step_bytes[i] = st.bytes;
+    macs += m;
+    bytes += st.bytes;
+  }
+  if (total) {
+    total->macs = macs;
+    total->bytes = bytes;
+    total->steps = (int64_t)p->steps.size();
+    total->launches = p->launches;
+  }
+  return PQ_OK;
+}
+
+int pq_plan_set_bitstrings(pq_ctx* c, pq_plan* p, int n, const uint64_t* bits) {
+  if (!c || !p) return PQ_ERR_ARG;
+  if (p->outq.empty()) return fail(c, PQ_ERR_ARG, "the plan has no bitstring batch");
+  const int64_t nb = (int64_t)1 << p->n_out_bits;
+  if (n < 1 || (int64_t)n > nb || !bits) return fail(c, PQ_ERR_ARG, "bitstring count out of range");
+  const size_t es = esize_of(c->dtype);
+  std::vector<unsigned char> col((size_t)(2 * nb) * es);
+  for (const auto& q : p->outq) {
+    std::fill(col.begin(), col.end(), 0);
+    for (int64_t b = 0; b < nb; ++b) {
+      const uint64_t w = bits[b < n ? b : 0];
+      const int64_t e = 2 * b + (int64_t)((w >> q.bit) & 1);   // column b: [1, 0] or [0, 1]
+      if (c->dtype == PQ_C128)
+        reinterpret_cast<double*>(col.data())[2 * e] = 1.0;
+      else
+        reinterpret_cast<float*>(col.data())[2 * e] = 1.0f;
+    }
+    const Slot& sl = p->slots[q.slot];
+    void* dst = (unsigned char*)sl.leaf_buf->ptr + sl.leaf_off * (int64_t)es;
+    hipError_t e = hipMemcpyAsync(dst, col.data(), col.size(), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hipfail(c, e, "bitstring upload");
+  }
+  return PQ_OK;
+}
+
+int pq_plan_run(pq_ctx* c, pq_plan* p, int n, const uint64_t* bits, double* out) {
+  if (!c || !p) return PQ_ERR_ARG;
+  if (n > 0) {
+    const int r = pq_plan_set_bitstrings(c, p, n, bits);
+    if (r) return r;
+  }
+  int r = pq_plan_execute(c, p, 1);
+  if (r) return r;
+  return out ? pq_plan_result(c, p, out) : PQ_OK;
 }
 
 int pq_plan_destroy(pq_ctx* c, pq_plan* p) {

@@ -159,4 +159,32 @@ function view_tensor!(b::HIPBackend, view_node, node, bond_idx, bond_range)
     pq_set!(b, Symbol(view_node), out[])
 end
 
+# Batched path (no reference counterpart): a compiled plan (pq_plan_create,
+# built by the host-side lowering) runs every slice of a sliced network and a
+# batch of output bitstrings in one graph replay.
+
+"""plan_run!(b, plan, bitstrings) -> Vector{ComplexF64}: bitstring words
+(bit q = closed output qubit q's value, as add_output! takes them per
+bitstring, src/layer3.jl:292-313), one run, the FP64 amplitudes of every
+bitstring (out_elems each, pq_plan_stats)."""
+function plan_run!(b::HIPBackend, plan::Ptr{Cvoid}, bitstrings::Vector{UInt64})
+    n_out = Ref{Int64}(0)
+    pq_check(b, ccall((:pq_plan_stats, LIBPQ), Cint,
+                      (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Ref{Int64}),
+                      plan, C_NULL, C_NULL, C_NULL, C_NULL, n_out))
+    out = Vector{Float64}(undef, 2 * n_out[])
+    pq_check(b, ccall((:pq_plan_run, LIBPQ), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{UInt64}, Ptr{Float64}),
+                      b.ctx, plan, length(bitstrings), bitstrings, out))
+    [complex(out[2i - 1], out[2i]) for i in 1:n_out[]]
+end
+
+"""plan_cost(b, plan) -> (macs, bytes): exact multiply-adds of one run, as
+the Metrics ledger counts them (src/backends.jl:38-55), and algorithmic bytes."""
+function plan_cost(b::HIPBackend, plan::Ptr{Cvoid})
+    c = Ref{NTuple{4, Int64}}((0, 0, 0, 0))   # pq_cost: two doubles, two int64
+    pq_check(b, ccall((:pq_plan_cost, LIBPQ), Cint, (Ptr{Cvoid}, Ref{NTuple{4, Int64}}, Ptr{Float64}, Ptr{Float64}),
+                      plan, c, C_NULL, C_NULL))
+    (reinterpret(Float64, c[][1]), reinterpret(Float64, c[][2]))
+end
+
 export HIPBackend

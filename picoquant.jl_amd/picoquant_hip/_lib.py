@@ -35,6 +35,11 @@ PQ_CHAIN_MAX_TILE_BITS = 13
 PQ_CHAIN_MAX_OP_BITS = 5
 PQ_CHAIN_STAGE_BYTES = 16384
 
+class Cost(ct.Structure):
+    """pq_cost (include/pq_hip.h)."""
+    _fields_ = [("macs", ct.c_double), ("bytes", ct.c_double), ("steps", ct.c_int64), ("launches", ct.c_int64)]
+
+
 # Every exported symbol, with (restype, argtypes).  tests/test_boundary.py
 # checks this list against include/pq_hip.h.
 _P = ct.c_void_p
@@ -77,6 +82,9 @@ SIGNATURES = {
                                      ct.POINTER(ct.c_int), ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
     "pq_plan_profile": (ct.c_int, [_P, _P, ct.POINTER(ct.c_double)]),
     "pq_plan_destroy": (ct.c_int, [_P, _P]),
+    "pq_plan_cost": (ct.c_int, [_P, _P, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
+    "pq_plan_set_bitstrings": (ct.c_int, [_P, _P, ct.c_int, ct.POINTER(ct.c_uint64)]),
+    "pq_plan_run": (ct.c_int, [_P, _P, ct.c_int, ct.POINTER(ct.c_uint64), ct.POINTER(ct.c_double)]),
     "pq_time_step": (ct.c_int, [_P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _I64P, ct.c_int, ct.c_int, ct.c_int,
                                 ct.c_int, ct.POINTER(ct.c_double)]),
     "pq_mlp_create": (ct.c_int, [_P, ct.c_int, ct.c_int, ct.c_uint64, ct.POINTER(_P)]),

@@ -50,6 +50,9 @@ class PlanDesc(ct.Structure):
         ("n_chain_words", ct.c_int64),
         ("chain_words", ct.POINTER(ct.c_int32)),
         ("n_out_bits", ct.c_int32),
+        ("n_out_qubits", ct.c_int32),
+        ("out_qubit_slot", ct.POINTER(ct.c_int32)),
+        ("out_qubit_bit", ct.POINTER(ct.c_int32)),
     ]
 
 
@@ -548,11 +551,17 @@ class Plan:
     def result(self):
         """The FP64 result in the output qubits' order; with a bitstring
         batch, shape (number of bitstrings,) + that shape."""
-        n = int(np.prod(self.out_shape)) if self.out_shape else 1
-        nb = 1 << self.n_out_bits
-        buf = np.empty(2 * n * nb, dtype=np.float64)
+        buf = self._result_buffer()
         self.ctx.check(self.ctx.lib.pq_plan_result(self.ctx.ptr, self.ptr,
                                                    buf.ctypes.data_as(ct.POINTER(ct.c_double))))
+        return self._shaped(buf)
+
+    def _result_buffer(self):
+        n = int(np.prod(self.out_shape)) if self.out_shape else 1
+        return np.empty(2 * n * (1 << self.n_out_bits), dtype=np.float64)
+
+    def _shaped(self, buf):
+        n = int(np.prod(self.out_shape)) if self.out_shape else 1
         z = buf[0::2] + 1j * buf[1::2]
         if self.n_bitstrings is None:
             return z.reshape(self.out_shape, order="F")
@@ -560,9 +569,13 @@ class Plan:
                          for b in range(self.n_bitstrings)])
 
     def set_bitstrings(self, bitstrings):
-        """Rewrite the output batch in place (pq_tensor_write): the next runs,
-        graph replays included, compute these bitstrings.  Same count as
-        the plan was compiled with."""
+        """Rewrite the output batch in place (pq_plan_set_bitstrings): the
+        next runs, graph replays included, compute these bitstrings.  Same
+        count as the plan was compiled with."""
+        words = self._words(bitstrings)
+        self.ctx.check(self.ctx.lib.pq_plan_set_bitstrings(self.ctx.ptr, self.ptr, len(words), words))
+
+    def _words(self, bitstrings):
         if self.n_bitstrings is None:
             raise ValueError("the plan was compiled without a bitstring batch")
         bitstrings = list(bitstrings)
@@ -571,8 +584,27 @@ class Plan:
         data, _ = output_batch(self.network, bitstrings)
         if set(data) != set(self.batch_handles):
             raise ValueError("the bitstrings close other outputs than the plan's")
-        for lab, col in data.items():
-            self.ctx.write(self.batch_handles[lab], col)
+        return bitstring_words(self.network, bitstrings)
+
+    def run(self, bitstrings=None):
+        """pq_plan_run: set the bitstrings (if given), run once (graph) and
+        return the result as result() does."""
+        words = self._words(bitstrings) if bitstrings is not None else None
+        buf = self._result_buffer()
+        self.ctx.check(self.ctx.lib.pq_plan_run(self.ctx.ptr, self.ptr, 0 if words is None else len(words), words,
+                                                buf.ctypes.data_as(ct.POINTER(ct.c_double))))
+        return self._shaped(buf)
+
+    def cost(self):
+        """pq_plan_cost: exact MACs and algorithmic bytes of one run, in
+        total and per step."""
+        tot = _lib.Cost()
+        m = np.zeros(len(self.steps), dtype=np.float64)
+        b = np.zeros(len(self.steps), dtype=np.float64)
+        self.ctx.check(self.ctx.lib.pq_plan_cost(self.ptr, ct.byref(tot), m.ctypes.data_as(ct.POINTER(ct.c_double)),
+                                                 b.ctypes.data_as(ct.POINTER(ct.c_double))))
+        return {"macs": tot.macs, "bytes": tot.bytes, "steps": tot.steps, "launches": tot.launches,
+                "step_macs": m, "step_bytes": b}
 
     def stats(self):
         a, l, o = ct.c_int64(), ct.c_int64(), ct.c_int64()
@@ -692,6 +724,35 @@ def output_batch(network, bitstrings):
     return data, n_bits
 
 
+def closed_output_qubits(network):
+    """[(node label, qubit index)] of the closed output qubits (the nodes
+    add_output made), qubit index = position in add_output's string."""
+    out = []
+    for i in range(network.number_qubits):
+        pos = network.qubit_ordering[i]
+        node = network.edges[network.output_qubits[pos - 1]].dst
+        if node is not None and network.node_layers.get(node) == -1:
+            out.append((node, i))
+    return out
+
+
+def bitstring_words(network, bitstrings):
+    """uint64 words of a bitstring batch for pq_plan_set_bitstrings: bit i
+    of word b = the value of closed output qubit i in bitstring b."""
+    closed = closed_output_qubits(network)
+    if any(i > 63 for _, i in closed):
+        raise ValueError("bitstring words hold at most 64 qubits")
+    words = (ct.c_uint64 * len(bitstrings))()
+    for b, bits in enumerate(bitstrings):
+        w = 0
+        for _, i in closed:
+            if bits[i] not in "01":
+                raise ValueError(f"bitstring {b} has {bits[i]!r} at closed qubit {i + 1}")
+            w |= int(bits[i]) << i
+        words[b] = w
+    return words
+
+
 def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=None, tiles=None,
                  budget_bytes=64 << 30, precision="uniform", fp64_steps=(), carry_elems=SMALL_ELEMS,
                  small_elems=MIXED_FP64_ELEMS, fuse=True, bitstrings=None):
@@ -773,10 +834,20 @@ def compile_plan(network, plan, sliced=(), out_order=None, group_bits=None, kc=N
         nmodes=_arr(ct.c_int32, nmodes), modes=_arr(ct.c_int64, modes or [0]),
         dt=_arr(ct.c_int32, dts), kind=_arr(ct.c_int32, kinds), arith=_arr(ct.c_int32, ariths),
         chain_off=_arr(ct.c_int64, chain_off), words=_arr(ct.c_int32, words or [0]))
+    # closed output qubits of a bitstring batch: their leaf slots and bits
+    oq_slot, oq_bit = [], []
+    if batch_handles:
+        slot_of = {s.leaf: i for i, s in enumerate(slots) if s.leaf}
+        for node, i in closed_output_qubits(network):
+            if node in batch_handles:
+                oq_slot.append(slot_of[batch_handles[node]])
+                oq_bit.append(i)
+    keep["oq_slot"] = _arr(ct.c_int32, oq_slot)
+    keep["oq_bit"] = _arr(ct.c_int32, oq_bit)
     d = PlanDesc(len(slots), keep["slot_elems"], keep["slot_mask"], keep["slot_leaf"], n_sliced, keep["lss"],
                  group_bits, len(steps), keep["step_slots"], keep["kc"], keep["tile"], keep["nmodes"],
                  keep["modes"], out_slot, keep["dt"], keep["kind"], keep["arith"], keep["chain_off"],
-                 len(words), keep["words"], n_out_bits)
+                 len(words), keep["words"], n_out_bits, len(oq_slot), keep["oq_slot"], keep["oq_bit"])
     p = ct.c_void_p()
     try:
         ctx.check(ctx.lib.pq_plan_create(ctx.ptr, ct.cast(ct.pointer(d), ct.c_void_p), ct.byref(p)))

@@ -70,6 +70,44 @@ def test_grid5x5_committed_bitstrings(golden_dir, precision, tol):
     p.close()
 
 
+def test_plan_run_and_cost_through_the_c_abi(golden_dir):
+    """pq_plan_run with raw 64-bit bitstring words (the boundary's one-call
+    batched path: set the batch, replay, FP64 result) gives the committed
+    config-2 amplitudes, also for a batch shorter than the compiled one
+    (padded on the device); pq_plan_cost counts the plan's exact MACs."""
+    import ctypes as ct
+    g = json.load(open(golden_dir / "grid5x5_d12.json"))
+    bits = g["bitstrings"]
+    want = np.array([complex(a, b) for a, b in g["amplitudes"]])
+    tn = _net(5, 5, 12, g["seed"], bits[0])
+    plan = PL.sweep_plan(tn, PL.grid_site_order(5, 5, "col"))
+    sliced = PL.choose_slices(tn, plan, 4)[0]
+    p = compile_plan(tn, plan, sliced, group_bits=2, precision="uniform", bitstrings=bits)
+    ctx, lib = p.ctx, p.ctx.lib
+    words = (ct.c_uint64 * 8)(*[sum(int(c) << i for i, c in enumerate(b)) for b in bits])
+    n_out = int(np.prod(p.out_shape)) if p.out_shape else 1
+    assert n_out == 1
+    out = np.empty(2 * 8, dtype=np.float64)
+    order = [3, 1, 4, 1, 5, 0, 2, 6]
+    for sel in (list(range(8)), order):
+        w = (ct.c_uint64 * 8)(*[words[i] for i in sel])
+        ctx.check(lib.pq_plan_run(ctx.ptr, p.ptr, 8, w, out.ctypes.data_as(ct.POINTER(ct.c_double))))
+        assert _rel(out[0::2] + 1j * out[1::2], want[sel]) < 1e-12
+    # three bitstrings: the other five results repeat the first
+    w = (ct.c_uint64 * 3)(words[6], words[2], words[5])
+    ctx.check(lib.pq_plan_run(ctx.ptr, p.ptr, 3, w, out.ctypes.data_as(ct.POINTER(ct.c_double))))
+    assert _rel(out[0::2] + 1j * out[1::2], want[[6, 2, 5, 6, 6, 6, 6, 6]]) < 1e-12
+    # a bad count is an error, not a run
+    w9 = (ct.c_uint64 * 9)()
+    assert lib.pq_plan_run(ctx.ptr, p.ptr, 9, w9, None) != 0
+    # cost: the steps' MACs, every step's MACs per slice times the slices it runs for
+    c = p.cost()
+    assert c["steps"] == len(p.steps) and c["macs"] == pytest.approx(float(np.sum(c["step_macs"])))
+    assert c["macs"] == pytest.approx(p.stats()["alg_flops"] / 8, rel=1e-12)
+    assert c["launches"] == p.stats()["launches"]
+    p.close()
+
+
 def test_bitstring_batch_with_open_qubits():
     """Five bitstrings (padded to 8 on the device) over a 3x3 grid whose
     qubits 2, 4 and 7 stay open: 5 x 8 amplitudes."""

@@ -213,3 +213,22 @@ def test_lowered_bitstring_batch(nslice, carry):
             full[open_q[1] - 1] = str((j >> 1) & 1)
             want = sv[sum(int(c) << q for q, c in enumerate(full))]
             assert abs(total[j] - want) < 1e-13
+
+
+def test_bitstring_words_for_the_c_abi():
+    """The uint64 words pq_plan_set_bitstrings reads: bit i = the value of
+    closed output qubit i (add_output's order), open qubits ignored; the
+    closed qubits are exactly those add_output closed."""
+    from picoquant_hip.executor import bitstring_words, closed_output_qubits
+    cfg = "1x0x11x01"
+    tn, _, _ = build(3, 3, 4, 5, cfg)
+    closed = closed_output_qubits(tn)
+    assert [i for _, i in closed] == [i for i, c in enumerate(cfg) if c != "x"]
+    assert len({node for node, _ in closed}) == len(closed)
+    batch = ["1x0x11x01", "0x1x00x10", "1x1x10x11"]
+    words = bitstring_words(tn, batch)
+    for b, bits in enumerate(batch):
+        want = sum(int(c) << i for i, c in enumerate(bits) if c != "x")
+        assert words[b] == want
+    with pytest.raises(ValueError):
+        bitstring_words(tn, ["1x0x11x0x"])
