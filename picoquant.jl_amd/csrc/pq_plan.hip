@@ -32,13 +32,8 @@ using namespace pqrt;
 
 namespace {
 
-__global__ void k_fill_zero(double* p, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = 0.0;
-}
-
-// out[j] += sum_b C[b * stride + j] for b < nb, in FP64 and in a fixed
-// order (the same bits on every run).  A workgroup of 256 lanes takes tj
+// out[j] = (overwrite ? 0 : out[j]) + sum_b C[b * stride + j] for b < nb,
+// in FP64 and in a fixed order (the same bits on every run).  A workgroup of 256 lanes takes tj
 // consecutive outputs with tb = 256 / tj lanes each (tj a power of two,
 // chosen by the host: few outputs with many copies give one output 256
 // lanes, many outputs give 256 outputs one lane each): lane (b0, j) sums
@@ -47,7 +42,7 @@ __global__ void k_fill_zero(double* p, int64_t n) {
 // pairwise in LDS.
 template <typename C>
 __global__ __launch_bounds__(256) void k_accum(const C* __restrict__ src, int64_t elems, int64_t stride, int nb,
-                                               int log_tj, double* __restrict__ out) {
+                                               int log_tj, int overwrite, double* __restrict__ out) {
   __shared__ double pr[256], pi[256];
   const int tj = 1 << log_tj, tb = 256 >> log_tj;
   const int lj = threadIdx.x & (tj - 1), b0 = threadIdx.x >> log_tj;
@@ -71,8 +66,9 @@ __global__ __launch_bounds__(256) void k_accum(const C* __restrict__ src, int64_
     __syncthreads();
   }
   if (b0 == 0 && j < elems) {
-    out[2 * j] += pr[lj];
-    out[2 * j + 1] += pi[lj];
+    // the first slice group of an output batch writes it, the others add
+    out[2 * j] = (overwrite ? 0.0 : out[2 * j]) + pr[lj];
+    out[2 * j + 1] = (overwrite ? 0.0 : out[2 * j + 1]) + pi[lj];
   }
 }
 
@@ -649,10 +645,8 @@ const char* parse_chain(pq_ctx* c, pq_plan* p, const pq_plan_desc* d, int i, PSt
 }
 
 int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
-  int64_t nb = p->out_elems * p->n_out;
-  unsigned g = (unsigned)std::min<int64_t>((nb + 255) / 256, 1024);
-  hipLaunchKernelGGL(k_fill_zero, dim3(std::max(1u, g)), dim3(256), 0, c->stream, p->out_dev, 2 * nb);
-  if (hipGetLastError() != hipSuccess) return fail(c, PQ_ERR_HIP, "fill kernel");
+  // (no fill of the FP64 result: the first slice group of every output
+  // batch writes it, k_accum below)
   auto do_launch = [&](const Launch& L, int64_t group) -> int {
     if (!L.grouped) {
       const int i = L.step;
@@ -773,17 +767,20 @@ int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
       if (r) return r;
     }
     const void* src = slot_ptr(c, p, p->out_slot, grp);
-    // the group's output batch: its bits above the summed slice bonds
-    double* dst = p->out_dev + 2 * p->out_elems * (grp >> (p->n_sliced - p->n_out_bits - p->group_bits));
+    // the group's output batch: its bits above the summed slice bonds; the
+    // batch's first group writes it
+    const int summed = p->n_sliced - p->n_out_bits - p->group_bits;
+    double* dst = p->out_dev + 2 * p->out_elems * (grp >> summed);
+    const int first = (grp & (((int64_t)1 << summed) - 1)) == 0 ? 1 : 0;
     const int lt = accum_log_tj(os.elems, nbatch);
     const unsigned ga = (unsigned)((os.elems + (1 << lt) - 1) >> lt);
     ProfScope ps(c, PQ_KC_ACCUM, (double)os.elems * nbatch * esize_of(os.dtype) + 16.0 * os.elems, 0);
     if (os.dtype == PQ_C128)
       hipLaunchKernelGGL((k_accum<c128>), dim3(ga), dim3(256), 0, c->stream, (const c128*)src,
-                         os.elems, os.elems, nbatch, lt, dst);
+                         os.elems, os.elems, nbatch, lt, first, dst);
     else
       hipLaunchKernelGGL((k_accum<c64>), dim3(ga), dim3(256), 0, c->stream, (const c64*)src,
-                         os.elems, os.elems, nbatch, lt, dst);
+                         os.elems, os.elems, nbatch, lt, first, dst);
     if (hipGetLastError() != hipSuccess) return fail(c, PQ_ERR_HIP, "accumulate kernel");
   }
   return PQ_OK;
@@ -1377,7 +1374,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
       return fail(c, PQ_ERR_OOM, "fused-chain table upload failed");
     }
   }
-  p->launches = 1 + (int64_t)p->seq1.size() + p->n_groups * ((int64_t)p->seq2.size() + 1);
+  p->launches = (int64_t)p->seq1.size() + p->n_groups * ((int64_t)p->seq2.size() + 1);
   // Streams: independent launches of a phase run concurrently
   // (pq_set_plan_streams; 1 = every launch on the context's stream).
   p->nstreams = std::max(1, std::min(c->plan_streams, PQ_PLAN_STREAMS));
