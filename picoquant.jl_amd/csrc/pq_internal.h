@@ -59,6 +59,9 @@ struct MultiDesc {
   int64_t hA[PQ_MAX_HIGH_BITS], hB[PQ_MAX_HIGH_BITS], hC[PQ_MAX_HIGH_BITS];
 };
 
+// K offsets of a grouped problem are staged in LDS up to this K (larger K
+// decodes each offset in the loop)
+#define PQ_MULTI_KMAX 64
 template <typename C>
 hipError_t launch_multi(const MultiDesc* descs, int n, unsigned blocks_x, unsigned batch, int64_t group,
                         hipStream_t s);

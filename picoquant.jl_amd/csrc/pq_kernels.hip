@@ -246,16 +246,48 @@ __global__ __launch_bounds__(256) void k_multi(const MultiDesc* __restrict__ des
   C* Out = reinterpret_cast<C*>(d.C) + oc;
   const uint32_t Mt = d.M.total, Nt = d.N.total, Kt = d.K.total;
   const uint32_t total = Mt * Nt;
+  // the problem's K offsets in LDS (grouped problems have K <= PQ_MULTI_KMAX),
+  // so a lane issues the loads of eight K values at once instead of waiting
+  // for each before decoding the next
+  __shared__ int2 kof[PQ_MULTI_KMAX];
+  const bool tab = Kt <= PQ_MULTI_KMAX;
+  if (tab) {
+    for (uint32_t k = threadIdx.x; k < Kt; k += blockDim.x) {
+      int32_t ka, kb;
+      group_offsets(d.K, k, ka, kb);
+      kof[k] = make_int2(ka, kb);
+    }
+    __syncthreads();
+  }
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     uint32_t m = t % Mt, n = t / Mt;
     int32_t pa, pc1, pb, pc2;
     group_offsets(d.M, m, pa, pc1);
     group_offsets(d.N, n, pb, pc2);
     C acc = czero<C>();
-    for (uint32_t k = 0; k < Kt; ++k) {
-      int32_t ka, kb;
-      group_offsets(d.K, k, ka, kb);
-      cfma(acc, A[pa + ka], B[pb + kb]);
+    uint32_t k = 0;
+    if (tab) {
+      for (; k + 8 <= Kt; k += 8) {
+        C a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int2 o = kof[k + u];
+          a[u] = A[pa + o.x];
+          b[u] = B[pb + o.y];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) cfma(acc, a[u], b[u]);
+      }
+      for (; k < Kt; ++k) {
+        const int2 o = kof[k];
+        cfma(acc, A[pa + o.x], B[pb + o.y]);
+      }
+    } else {
+      for (; k < Kt; ++k) {
+        int32_t ka, kb;
+        group_offsets(d.K, k, ka, kb);
+        cfma(acc, A[pa + ka], B[pb + kb]);
+      }
     }
     Out[pc1 + pc2] = acc;
   }

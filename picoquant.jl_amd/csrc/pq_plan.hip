@@ -1030,9 +1030,17 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     return st.kind == 1 && p->slots[st.a].dtype == PQ_C128 && p->slots[st.c].dtype == PQ_C64 &&
            conv_elems(st) <= 65536;
   };
+  // Grouped: contractions of at most 2^17 outputs and K <= PQ_MULTI_KMAX,
+  // whatever kernel they would run alone (small GEMMs too: two or more of
+  // them per level share one launch instead of one launch each; on the 6x6
+  // depth-14 plan 49 -> 36 launches and 1.40 -> 1.33 ms; at 2^18 / 2^19
+  // outputs the grouped kernel's per-output loop loses more than the saved
+  // launches: 1.38 / 1.50 ms).
   auto small = [&](const PStep& st) {
-    return (st.kind == 0 && (st.spec.kc == PQ_KC_STREAM || st.spec.kc == PQ_KC_GENERIC) &&
-            st.spec.Mt * st.spec.Nt <= 65536 && st.spec.Kt <= 64) ||
+    return (st.kind == 0 &&
+            (st.spec.kc == PQ_KC_STREAM || st.spec.kc == PQ_KC_GENERIC ||
+             (st.spec.kc == PQ_KC_GEMM && st.spec.splitk <= 1)) &&
+            st.spec.Mt * st.spec.Nt <= (1u << 17) && st.spec.Kt <= PQ_MULTI_KMAX) ||
            small_conv(st);
   };
   auto build_seq = [&](const std::vector<int>& order, std::vector<Launch>& seq) {
