@@ -11,8 +11,10 @@
 //      lanes read adjacent addresses),
 //   2. apply the operators in order, in place:
 //        T[r, m] = sum_k S_t[m, k] T[r, k]
-//      as 16 x 16 x 4 complex blocks on v_mfma_f32_16x16x4_f32 (4 real
-//      MFMAs per block: Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br),
+//      as 16 x 16 x 4 complex blocks on v_mfma_f32_16x16x4_f32, with
+//      three real products per complex one (Gauss): P1 = Ar Br, P2 = Ai Bi,
+//      P3 = (Ar + Ai)(Br + Bi), each summed over K in an MFMA accumulator,
+//      then Cr = P1 - P2, Ci = P3 - P1 - P2,
 //   3. scatter it to Y (bits ordered by Y stride).
 // Workgroups are persistent (two per CU, 8 waves each, one tile buffer
 // each): while the stages run on one tile, the next tile's loads are in
@@ -61,11 +63,15 @@ __device__ __forceinline__ floatx4 mfma(float a, float b, floatx4 c) {
 // w + CH_THREADS / 64, ...  The lane's operator values (MFMA A operand:
 // row m = 16 mb + li, column k = 4 kc + lg) stay in registers when there
 // are at most 8 chunk-blocks of them; the 32 x 32 case rereads them from
-// LDS per M block (register room for the next tile's prefetch).
+// LDS per M block (register room for the next tile's prefetch).  With at
+// most 4 chunk-blocks (16 x 16 and smaller, 8 -> 32) a complex product takes
+// three real MFMAs (Gauss), otherwise four: the third accumulator and the
+// sums of the larger operators cost registers, and measured slower.
 template <int NKC, int NMB, bool FULL>
 __device__ __forceinline__ void stage(lc* tile, const lc* sd, const int* si_, const ChainStage& g, int wave, int li,
                                       int lg) {
   constexpr bool HOLD = NKC * NMB <= 8;
+  constexpr bool GAUSS = NKC * NMB <= 4;
   constexpr int HMB = HOLD ? NMB : 1;
   const int nk = 1 << g.nk, nm = 1 << g.nm;
   auto opval = [&](int kc, int mb) -> lc {
@@ -109,7 +115,8 @@ __device__ __forceinline__ void stage(lc* tile, const lc* sd, const int* si_, co
   // block's MFMAs run and its M values are written (blocks hold different
   // r's, so a block may be written before a later one is read).  A block
   // number past the end repeats the wave's first block and is not written.
-  constexpr int UNR = (NKC <= 4 && NMB == 1) ? 4 : (NKC * NMB <= 4 ? 2 : 1);
+  // 16 x 16 takes 2 blocks: with three accumulators per block, 4 spill.
+  constexpr int UNR = (NKC <= 2 && NMB == 1) ? 4 : (NKC * NMB <= 4 ? 2 : 1);
   for (int rb0 = wave * UNR; rb0 < nrb; rb0 += NW * UNR) {
     int roff[UNR];
     float br[UNR][NKC], bi[UNR][NKC];
@@ -142,22 +149,38 @@ __device__ __forceinline__ void stage(lc* tile, const lc* sd, const int* si_, co
           }
         }
         constexpr int hm = 0;
-        floatx4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+        floatx4 cr, ci;
+        if constexpr (GAUSS) {
+          // three real products per complex one
+          floatx4 p1 = {0.f, 0.f, 0.f, 0.f}, p2 = {0.f, 0.f, 0.f, 0.f}, p3 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kc = 0; kc < NKC; ++kc) {
-          const float ar = sr[kc][HOLD ? mb : hm], ai = si[kc][HOLD ? mb : hm];
-          cr = mfma(ar, br[u][kc], cr);
-          ci = mfma(ar, bi[u][kc], ci);
-          cr = mfma(-ai, bi[u][kc], cr);
-          ci = mfma(ai, br[u][kc], ci);
+          for (int kc = 0; kc < NKC; ++kc) {
+            p1 = mfma(sr[kc][mb], br[u][kc], p1);
+            p2 = mfma(si[kc][mb], bi[u][kc], p2);
+            p3 = mfma(sr[kc][mb] + si[kc][mb], br[u][kc] + bi[u][kc], p3);
+          }
+          cr = p1 - p2;
+          ci = (p3 - p1) - p2;
+        } else {
+          // four real products per complex one
+          cr = floatx4{0.f, 0.f, 0.f, 0.f};
+          ci = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int kc = 0; kc < NKC; ++kc) {
+            const float ar = sr[kc][HOLD ? mb : hm], ai = si[kc][HOLD ? mb : hm];
+            cr = mfma(ar, br[u][kc], cr);
+            ci = mfma(ar, bi[u][kc], ci);
+            cr = mfma(-ai, bi[u][kc], cr);
+            ci = mfma(ai, br[u][kc], ci);
+          }
         }
         // accumulator register i holds row 4 lg + i, column li
         if (wr) {
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             if (FULL || (rv && 16 * mb + 4 * lg + i < nm)) {
-              // real and imaginary parts written from the two accumulators
-              // as they are (one ds_write2_b32), no register pairing
+              // real and imaginary parts written as two 32-bit values (one
+              // ds_write2_b32), no register pairing
               float* e = &at(tile, roff[u] ^ mof[mb][i]).x;
               e[0] = cr[i];
               e[1] = ci[i];
@@ -173,7 +196,7 @@ __device__ __forceinline__ void stage(lc* tile, const lc* sd, const int* si_, co
 // access (a.gvec / a.svec); separate instantiations keep the two address
 // forms out of each other's code.
 template <bool GV, bool SV>
-__global__ __launch_bounds__(CH_THREADS) void k_chain(const ChainArgs a) {
+__global__ __launch_bounds__(CH_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_chain(const ChainArgs a) {
   extern __shared__ lc smem[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
