@@ -12,9 +12,6 @@
 //   Cr += Ar*Br + (-Ai)*Bi ;  Ci += Ar*Bi + Ai*Br
 // complex64  -> v_mfma_f32_16x16x4_f32 (fp32 accumulate)
 // complex128 -> v_mfma_f64_16x16x4_f64 (fp64 accumulate)
-// except in k_gemm2's complex64 path, which takes three (Gauss): per K tile
-// P1 = sum Ar*Br, P2 = sum Ai*Bi, P3 = sum (Ar+Ai)*(Br+Bi), then
-// Cr += P1 - P2, Ci += P3 - P1 - P2.
 //
 // Tile: 256 lanes = 4 waves arranged WGM x WGN (2x2, 4x1 or 1x4); workgroup
 // tile BM x BN, each wave (BM/WGM) x (BN/WGN) as a grid of 16x16 MFMA tiles;
@@ -372,10 +369,7 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
   const typename MF::acc zero = {};
   load_tile(0);
   for (uint32_t kt = 0; kt < ntiles; ++kt) {
-    // complex64: a K tile's sums of the three real products of Gauss's
-    // complex product, P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi)
-    // (Cr = P1 - P2, Ci = P3 - P1 - P2), three MFMAs per complex block
-    typename MF::acc p1[TM][TN], p2[TM][TN], p3[TM][TN];
+    typename MF::acc cr[TM][TN], ci[TM][TN];
     store_tile();
     __syncthreads();
     if (kt + 1 < ntiles) load_tile(kt + 1);
@@ -386,45 +380,21 @@ __global__ __launch_bounds__(256) void k_gemm2(const C* __restrict__ A, const C*
       for (int i = 0; i < TM; ++i) a[i] = Ap[16 * i * LDK + kk];
 #pragma unroll
       for (int j = 0; j < TN; ++j) b[j] = Bp[16 * j * LDK + kk];
-      if constexpr (CHUNKED) {
-        R as[TM], bs[TN];
-#pragma unroll
-        for (int i = 0; i < TM; ++i) as[i] = a[i].x + a[i].y;
-#pragma unroll
-        for (int j = 0; j < TN; ++j) bs[j] = b[j].x + b[j].y;
-        // a K tile's first products start the tile's fresh sums
-        const bool fresh = kk == 0;
-#pragma unroll
-        for (int i = 0; i < TM; ++i)
-#pragma unroll
-          for (int j = 0; j < TN; ++j) {
-            p1[i][j] = MF::op(a[i].x, b[j].x, fresh ? zero : p1[i][j]);
-            p2[i][j] = MF::op(a[i].y, b[j].y, fresh ? zero : p2[i][j]);
-            p3[i][j] = MF::op(as[i], bs[j], fresh ? zero : p3[i][j]);
-          }
-      } else {
-#pragma unroll
-        for (int i = 0; i < TM; ++i)
-#pragma unroll
-          for (int j = 0; j < TN; ++j) {
-            accr[i][j] = MF::op(a[i].x, b[j].x, accr[i][j]);
-            accr[i][j] = MF::op(-a[i].y, b[j].y, accr[i][j]);
-            acci[i][j] = MF::op(a[i].x, b[j].y, acci[i][j]);
-            acci[i][j] = MF::op(a[i].y, b[j].x, acci[i][j]);
-          }
-      }
-    }
-    if constexpr (CHUNKED) {
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            accr[i][j][r] += p1[i][j][r] - p2[i][j][r];
-            acci[i][j][r] += (p3[i][j][r] - p1[i][j][r]) - p2[i][j][r];
-          }
+        for (int j = 0; j < TN; ++j) {
+          typename MF::acc& xr = CHUNKED ? cr[i][j] : accr[i][j];
+          typename MF::acc& xi = CHUNKED ? ci[i][j] : acci[i][j];
+          // a K tile's first products start the tile's fresh sums
+          const bool fresh = CHUNKED && kk == 0;
+          xr = MF::op(a[i].x, b[j].x, fresh ? zero : xr);
+          xr = MF::op(-a[i].y, b[j].y, xr);
+          xi = MF::op(a[i].x, b[j].y, fresh ? zero : xi);
+          xi = MF::op(a[i].y, b[j].x, xi);
+        }
     }
+    if constexpr (CHUNKED) add_chunk<MF, TM, TN>(accr, acci, cr, ci);
     __syncthreads();
   }
 
