@@ -73,8 +73,9 @@ hipError_t launch_multi(const MultiDesc* descs, int n, unsigned blocks_x, unsign
 // ktab[kt .. kt + nk).  Conversions: item = (step, source offset, -, target
 // offset).  lv[b] is the offset in lv of tree b's entry: nlevels, its first
 // step record and number of step records, its first K table entry and
-// number of K entries (a tree's records and K entries are contiguous), then
-// the nlevels + 1 item boundaries of its levels.
+// number of K entries (a tree's records and K entries are contiguous), its
+// first leaf copy and number of leaf copies, then the nlevels + 1 item
+// boundaries of its levels.
 struct ForestStep {
   const void* A;
   const void* B;
@@ -82,12 +83,24 @@ struct ForestStep {
   int32_t nk, kt;     // K extent, first K table entry
   int32_t conv;       // 1: element-type conversion A -> C (the other type)
   int32_t c128;       // computes in (converts from) complex128
+  int32_t lds;        // bit 0 / 1 / 2: A / B / C is in the tree's LDS data area,
+                      // the pointer holding its byte offset there
 };
-// lds_bytes: LDS for the largest tree's step records and K offsets (0: the
-// kernel reads them from global memory), at most PQ_FOREST_LDS_BYTES
+// Leaf operands copied into a tree's LDS data area when its workgroup
+// starts, 8 bytes per entry: source address and byte offset in the area.
+struct ForestCopy {
+  const void* src;
+  int32_t dst, pad;
+};
+// A tree's LDS: its step records (32-bit words, padded to 8 bytes), its K
+// offsets, then (16-byte aligned) its data area.  Per-lane leaf copies:
+#define PQ_FOREST_COPY_PER_LANE 8
+// lds_bytes: LDS of the largest tree (0: the kernel reads records and K
+// offsets from global memory and keeps every operand there), at most
+// PQ_FOREST_LDS_BYTES
 #define PQ_FOREST_LDS_BYTES (48 << 10)
-hipError_t launch_forest(const ForestStep* steps, const int4* items, const int2* ktab, const int32_t* lv,
-                         int ntrees, size_t lds_bytes, hipStream_t s);
+hipError_t launch_forest(const ForestStep* steps, const int4* items, const int2* ktab, const ForestCopy* copies,
+                         const int32_t* lv, int ntrees, size_t lds_bytes, hipStream_t s);
 
 template <typename C>
 hipError_t launch_stream(const C* X, const C* Y, C* Out, const StreamArgs& a, hipStream_t s);

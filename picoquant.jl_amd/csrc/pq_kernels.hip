@@ -297,18 +297,22 @@ __global__ __launch_bounds__(256) void k_multi(const MultiDesc* __restrict__ des
 // One item: an output element of a small unbatched step (one lane, the K
 // offsets from the step's table), or one element of a conversion.
 template <typename C>
-__device__ __forceinline__ void forest_item(const ForestStep& f, const int4& it, const int2* ktab) {
+__device__ __forceinline__ void forest_item(const ForestStep& f, const int4& it, const int2* ktab, char* dat) {
+  // an operand in the tree's LDS data area holds its byte offset there
+  auto at = [&](const void* q, int bit) -> char* {
+    return ((f.lds >> bit) & 1) ? dat + (size_t)q : reinterpret_cast<char*>(const_cast<void*>(q));
+  };
   if (f.conv) {
     typedef typename std::conditional<sizeof(C) == 16, c64, c128>::type D;
-    const C v = reinterpret_cast<const C*>(f.A)[it.y];
+    const C v = reinterpret_cast<const C*>(at(f.A, 0))[it.y];
     D w;
     w.x = (decltype(w.x))v.x;
     w.y = (decltype(w.y))v.y;
-    reinterpret_cast<D*>(f.C)[it.w] = w;
+    reinterpret_cast<D*>(at(f.C, 2))[it.w] = w;
     return;
   }
-  const C* A = reinterpret_cast<const C*>(f.A) + it.y;
-  const C* B = reinterpret_cast<const C*>(f.B) + it.z;
+  const C* A = reinterpret_cast<const C*>(at(f.A, 0)) + it.y;
+  const C* B = reinterpret_cast<const C*>(at(f.B, 1)) + it.z;
   C acc = czero<C>();
   // the loads of several K values in flight at once (a tree's level waits
   // for its slowest lane)
@@ -317,7 +321,7 @@ __device__ __forceinline__ void forest_item(const ForestStep& f, const int4& it,
     const int2 o = ktab[k];
     cfma(acc, A[o.x], B[o.y]);
   }
-  reinterpret_cast<C*>(f.C)[it.w] = acc;
+  reinterpret_cast<C*>(at(f.C, 2))[it.w] = acc;
 }
 
 // Workgroup b runs tree b level by level: a level's items (the outputs of
@@ -325,25 +329,51 @@ __device__ __forceinline__ void forest_item(const ForestStep& f, const int4& it,
 // and the barrier after each level makes its stores visible to the
 // workgroup's later levels (one workgroup runs on one CU).  Tree header at
 // lv[lv[b]]: levels, first step record, step records, first K entry, K
-// entries, then the levels' item bounds.  LDS: the tree's step records and K
-// offsets are copied into LDS once, so a level waits on global memory only
-// for its operands (the items of the next level are read ahead, as they
-// depend on nothing the level computes).
+// entries, first leaf copy, leaf copies, then the levels' item bounds.
+// LDS: the tree's step records and K offsets are copied into LDS once, so
+// a level waits on global memory only for its operands (the items of the
+// next level are read ahead, as they depend on nothing the level computes);
+// the results that the tree itself reads later, and the small leaves it
+// reads, live in the LDS data area after them (leaves copied in at the
+// start), so only the other leaves and the tree's root result go through
+// global memory.
 template <bool LDS>
 __global__ __launch_bounds__(256) void k_forest(const ForestStep* __restrict__ fs, const int4* __restrict__ items,
-                                                const int2* __restrict__ ktab, const int32_t* __restrict__ lv) {
-  extern __shared__ int32_t fsm[];
+                                                const int2* __restrict__ ktab,
+                                                const ForestCopy* __restrict__ copies,
+                                                const int32_t* __restrict__ lv) {
+  extern __shared__ __align__(16) int32_t fsm[];
   const int32_t q = lv[blockIdx.x];
   const int32_t nlev = lv[q], fs0 = lv[q + 1], nfs = lv[q + 2], k0 = lv[q + 3], nkt = lv[q + 4];
-  const int32_t* bnd = lv + q + 5;
+  const int32_t c0 = lv[q + 5], nc = lv[q + 6];
+  const int32_t* bnd = lv + q + 7;
   const ForestStep* F = fs + fs0;
   const int2* KT = ktab + k0;
+  char* dat = nullptr;
   if constexpr (LDS) {
     constexpr int W = (int)(sizeof(ForestStep) / 4);
+    const int tid = (int)threadIdx.x;
+    // leaf copies first: their loads are in flight while the records move
+    float2 cv[PQ_FOREST_COPY_PER_LANE];
+    int32_t cd[PQ_FOREST_COPY_PER_LANE];
+#pragma unroll
+    for (int j = 0; j < PQ_FOREST_COPY_PER_LANE; ++j) {
+      const int32_t e = tid + 256 * j;
+      if (e < nc) {
+        const ForestCopy cp = copies[c0 + e];
+        cv[j] = *reinterpret_cast<const float2*>(cp.src);
+        cd[j] = cp.dst;
+      }
+    }
     const int32_t* src = reinterpret_cast<const int32_t*>(fs + fs0);
-    for (int32_t e = (int32_t)threadIdx.x; e < nfs * W; e += (int32_t)blockDim.x) fsm[e] = src[e];
-    int2* kd = reinterpret_cast<int2*>(fsm + nfs * W + (nfs * W & 1));
-    for (int32_t e = (int32_t)threadIdx.x; e < nkt; e += (int32_t)blockDim.x) kd[e] = ktab[k0 + e];
+    for (int32_t e = tid; e < nfs * W; e += (int32_t)blockDim.x) fsm[e] = src[e];
+    const int32_t kw = nfs * W + (nfs * W & 1);
+    int2* kd = reinterpret_cast<int2*>(fsm + kw);
+    for (int32_t e = tid; e < nkt; e += (int32_t)blockDim.x) kd[e] = ktab[k0 + e];
+    dat = reinterpret_cast<char*>(fsm) + (((size_t)kw * 4 + (size_t)nkt * 8 + 15) & ~(size_t)15);
+#pragma unroll
+    for (int j = 0; j < PQ_FOREST_COPY_PER_LANE; ++j)
+      if (tid + 256 * j < nc) *reinterpret_cast<float2*>(dat + cd[j]) = cv[j];
     __syncthreads();
     F = reinterpret_cast<const ForestStep*>(fsm);
     KT = kd;
@@ -358,9 +388,9 @@ __global__ __launch_bounds__(256) void k_forest(const ForestStep* __restrict__ f
       const int4 it = i == i0 ? cur : items[i];
       const ForestStep& f = F[it.x - fs0];
       if (f.c128)
-        forest_item<c128>(f, it, KT + (f.kt - k0));
+        forest_item<c128>(f, it, KT + (f.kt - k0), dat);
       else
-        forest_item<c64>(f, it, KT + (f.kt - k0));
+        forest_item<c64>(f, it, KT + (f.kt - k0), dat);
     }
     __syncthreads();
   }
@@ -432,13 +462,13 @@ hipError_t launch_multi(const MultiDesc* descs, int n, unsigned blocks_x, unsign
   return hipGetLastError();
 }
 
-hipError_t launch_forest(const ForestStep* steps, const int4* items, const int2* ktab, const int32_t* lv,
-                         int ntrees, size_t lds_bytes, hipStream_t s) {
+hipError_t launch_forest(const ForestStep* steps, const int4* items, const int2* ktab, const ForestCopy* copies,
+                         const int32_t* lv, int ntrees, size_t lds_bytes, hipStream_t s) {
   if (ntrees < 1 || ntrees > (1 << 24) || lds_bytes > PQ_FOREST_LDS_BYTES) return hipErrorInvalidValue;
   if (lds_bytes > 0)
-    hipLaunchKernelGGL(k_forest<true>, dim3(ntrees), dim3(256), lds_bytes, s, steps, items, ktab, lv);
+    hipLaunchKernelGGL(k_forest<true>, dim3(ntrees), dim3(256), lds_bytes, s, steps, items, ktab, copies, lv);
   else
-    hipLaunchKernelGGL(k_forest<false>, dim3(ntrees), dim3(256), 0, s, steps, items, ktab, lv);
+    hipLaunchKernelGGL(k_forest<false>, dim3(ntrees), dim3(256), 0, s, steps, items, ktab, copies, lv);
   return hipGetLastError();
 }
 

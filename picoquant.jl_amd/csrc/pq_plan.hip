@@ -24,6 +24,7 @@
 #include "pq_rt.h"
 
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <functional>
 #include <set>
@@ -164,8 +165,9 @@ struct pq_plan {
   const ForestStep* f_steps = nullptr;
   const int4* f_items = nullptr;
   const int2* f_ktab = nullptr;
+  const ForestCopy* f_copies = nullptr;
   const int32_t* f_lv = nullptr;
-  size_t f_lds = 0;                  // forest launch: LDS bytes of its tree tables (0: none)
+  size_t f_lds = 0;                  // forest launch: LDS bytes of the largest tree (0: none)
   int n_sliced = 0, group_bits = 0;
   int64_t n_groups = 1;
   struct OutQubit {
@@ -706,7 +708,8 @@ int run_program(pq_ctx* c, pq_plan* p, bool profile_steps) {
     {
       ProfScope ps(c, PQ_KC_BATCHED, by, fl);
       if (L.forest)
-        e = launch_forest(p->f_steps, p->f_items, p->f_ktab, p->f_lv, (int)L.tree_start.size() - 1, p->f_lds,
+        e = launch_forest(p->f_steps, p->f_items, p->f_ktab, p->f_copies, p->f_lv, (int)L.tree_start.size() - 1,
+                          p->f_lds,
                           c->stream);
       else
         e = (L.dtype == PQ_C128)
@@ -1287,10 +1290,13 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
       }
     };
     std::vector<ForestStep> fsv;
+    std::vector<int> fstep;   // the plan step of each record
     std::vector<int4> items;
     std::vector<int2> ktab;
+    std::vector<ForestCopy> copies;
     const int nt = (int)L.tree_start.size() - 1;
-    std::vector<int32_t> lv(nt, 0);
+    std::vector<int32_t> lv(nt, 0), hdrs(nt);
+    std::vector<size_t> tbytes(nt);
     size_t f_lds = 0;
     for (int b = 0; b < nt; ++b) {
       lv[b] = (int32_t)lv.size();
@@ -1299,14 +1305,17 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
         lvl[p->steps[L.steps[q]].level].push_back(L.steps[q]);
       lv.push_back((int32_t)lvl.size());
       const size_t hdr = lv.size();
+      hdrs[b] = (int32_t)hdr;
       const int32_t fs0 = (int32_t)fsv.size(), k0 = (int32_t)ktab.size();
-      lv.insert(lv.end(), 4, 0);   // first step record, records, first K entry, K entries
+      // first step record, records, first K entry, K entries, first leaf copy, leaf copies
+      lv.insert(lv.end(), 6, 0);
       lv.push_back((int32_t)items.size());
       for (auto& kv : lvl) {
         for (int i : kv.second) {
           const PStep& st = p->steps[i];
           ForestStep f;
           memset(&f, 0, sizeof(f));
+          fstep.push_back(i);
           f.A = slot_ptr(c, p, st.a, 0);
           f.B = slot_ptr(c, p, st.b, 0);
           f.C = slot_ptr(c, p, st.c, 0);
@@ -1343,21 +1352,120 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
       // LDS of the tree's records and K offsets (k_forest: records as 32-bit
       // words, then the K entries 8-byte aligned)
       const size_t words = (size_t)nfs * (sizeof(ForestStep) / 4);
-      f_lds = std::max(f_lds, 4 * (words + (words & 1)) + sizeof(int2) * (size_t)nkt);
+      tbytes[b] = 4 * (words + (words & 1)) + sizeof(int2) * (size_t)nkt;
+      f_lds = std::max(f_lds, tbytes[b]);
     }
     if (items.size() >= ((size_t)1 << 31)) {
       free_plan(c, p);
       return fail(c, PQ_ERR_UNSUPPORTED, "forest too large");
     }
+    // With the tables in LDS, each tree also keeps in LDS (its data area,
+    // after the tables) the results that it reads itself and the small
+    // leaves that it reads, while they fit: a result lives from its level
+    // to its reader's level, a leaf from the start to its reader's level;
+    // first fit, 16-byte aligned.
+    if (f_lds <= PQ_FOREST_LDS_BYTES) {
+      // readers of each slot over the whole plan: only a result with one
+      // reader, in its own tree, and not the plan's output moves to LDS
+      std::vector<int> nread(p->slots.size(), 0);
+      for (const PStep& st : p->steps) {
+        ++nread[st.a];
+        if (st.kind == 0) ++nread[st.b];
+        for (int os : st.ops) ++nread[os];
+      }
+      nread[p->out_slot] += 1;
+      for (int b = 0; b < nt; ++b) {
+        const int32_t fs0 = lv[hdrs[b]], nfs = lv[hdrs[b] + 1];
+        const size_t base = (tbytes[b] + 15) & ~(size_t)15;
+        if (base >= PQ_FOREST_LDS_BYTES) continue;
+        const int64_t budget = (int64_t)(PQ_FOREST_LDS_BYTES - base);
+        // the record and operand (0: A, 1: B) reading each result of the tree
+        std::map<int, std::pair<int, int>> reader;
+        for (int r = fs0; r < fs0 + nfs; ++r) {
+          const PStep& st = p->steps[fstep[r]];
+          for (int o = 0; o < (st.kind == 1 ? 1 : 2); ++o) {
+            const int x = o ? st.b : st.a;
+            if (!p->slots[x].leaf && nread[x] == 1) reader[x] = {r, o};
+          }
+        }
+        struct Live {
+          int64_t off, end;
+          int last;   // level of the reader
+        };
+        std::vector<Live> live;
+        auto place = [&](int64_t bytes, int now, int last) -> int64_t {
+          live.erase(std::remove_if(live.begin(), live.end(), [&](const Live& a) { return a.last < now; }),
+                     live.end());
+          std::sort(live.begin(), live.end(), [](const Live& x, const Live& y) { return x.off < y.off; });
+          int64_t at = 0;
+          for (const Live& a : live) {
+            if (a.off - at >= bytes) break;
+            at = std::max(at, a.end);
+          }
+          if (at + bytes > budget) return -1;
+          live.push_back({at, at + ((bytes + 15) & ~(int64_t)15), last});
+          return at;
+        };
+        const int32_t c0 = (int32_t)copies.size();
+        int64_t pieces = 0, high = 0;
+        const int64_t max_pieces = 256 * PQ_FOREST_COPY_PER_LANE;
+        for (int r = fs0; r < fs0 + nfs; ++r) {
+          const PStep& st = p->steps[fstep[r]];
+          for (int o = 0; o < (st.kind == 1 ? 1 : 2); ++o) {
+            const Slot& sl = p->slots[o ? st.b : st.a];
+            if (!sl.leaf) continue;
+            bool fixed = true;   // read at one offset (keeps no sliced bond)
+            for (int64_t v : sl.slice_stride) fixed = fixed && v == 0;
+            // the elements the leaf's view reaches
+            const int64_t bytes = (sl.leaf_span + 1) * (sl.dtype == PQ_C128 ? 16 : 8);
+            if (!fixed || bytes > 4096 || pieces + bytes / 8 > max_pieces) continue;
+            const int64_t off = place(bytes, INT32_MIN, st.level);
+            if (off < 0) continue;
+            const char* src = (const char*)(o ? fsv[r].B : fsv[r].A);
+            for (int64_t w = 0; w < bytes / 8; ++w) copies.push_back({src + 8 * w, (int32_t)(off + 8 * w), 0});
+            pieces += bytes / 8;
+            high = std::max(high, off + bytes);
+            fsv[r].lds |= 1 << o;
+            if (o) fsv[r].B = (const void*)(uintptr_t)off;
+            else fsv[r].A = (const void*)(uintptr_t)off;
+          }
+        }
+        for (int r = fs0; r < fs0 + nfs; ++r) {
+          const PStep& st = p->steps[fstep[r]];
+          auto rd = reader.find(st.c);
+          if (rd == reader.end()) continue;   // the tree's root: read by a later launch
+          const Slot& sl = p->slots[st.c];
+          // every copy of a batched result (a forest step covers them all
+          // when the batch modes fold into its free modes)
+          const int64_t bytes = sl.elems * sl.copies() * (sl.dtype == PQ_C128 ? 16 : 8);
+          const int64_t off = place(bytes, st.level, p->steps[fstep[rd->second.first]].level);
+          if (off < 0) continue;
+          high = std::max(high, off + bytes);
+          fsv[r].lds |= 4;
+          fsv[r].C = (void*)(uintptr_t)off;
+          ForestStep& u = fsv[rd->second.first];
+          u.lds |= 1 << rd->second.second;
+          if (rd->second.second) u.B = (const void*)(uintptr_t)off;
+          else u.A = (const void*)(uintptr_t)off;
+        }
+        lv[hdrs[b] + 4] = c0;
+        lv[hdrs[b] + 5] = (int32_t)copies.size() - c0;
+        if (high > 0) f_lds = std::max(f_lds, base + (size_t)high);
+      }
+    }
     if (ktab.empty()) ktab.push_back(make_int2(0, 0));
+    if (copies.empty()) copies.push_back({nullptr, 0, 0});
     const size_t b_steps = fsv.size() * sizeof(ForestStep), b_items = items.size() * sizeof(int4),
-                 b_k = ktab.size() * sizeof(int2), b_lv = lv.size() * sizeof(int32_t);
+                 b_k = ktab.size() * sizeof(int2), b_cp = copies.size() * sizeof(ForestCopy),
+                 b_lv = lv.size() * sizeof(int32_t);
     auto up = [](size_t x) { return (x + 255) / 256 * 256; };
-    const size_t o_items = up(b_steps), o_k = o_items + up(b_items), o_lv = o_k + up(b_k), total = o_lv + up(b_lv);
+    const size_t o_items = up(b_steps), o_k = o_items + up(b_items), o_cp = o_k + up(b_k), o_lv = o_cp + up(b_cp),
+                 total = o_lv + up(b_lv);
     std::vector<unsigned char> blob(total, 0);
     memcpy(blob.data(), fsv.data(), b_steps);
     memcpy(blob.data() + o_items, items.data(), b_items);
     memcpy(blob.data() + o_k, ktab.data(), b_k);
+    memcpy(blob.data() + o_cp, copies.data(), b_cp);
     memcpy(blob.data() + o_lv, lv.data(), b_lv);
     if (hipMalloc(&p->forest_buf, total) != hipSuccess ||
         hipMemcpy(p->forest_buf, blob.data(), total, hipMemcpyHostToDevice) != hipSuccess) {
@@ -1369,6 +1477,7 @@ int pq_plan_create(pq_ctx* c, const pq_plan_desc* d, pq_plan** out) {
     p->f_steps = (const ForestStep*)fb;
     p->f_items = (const int4*)(fb + o_items);
     p->f_ktab = (const int2*)(fb + o_k);
+    p->f_copies = (const ForestCopy*)(fb + o_cp);
     p->f_lv = (const int32_t*)(fb + o_lv);
     // the tables in LDS when the largest tree's fit, else read from memory
     p->f_lds = f_lds <= PQ_FOREST_LDS_BYTES ? f_lds : 0;
