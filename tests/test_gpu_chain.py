@@ -1,14 +1,19 @@
 """Fused-chain steps (PQ_KC_CHAIN) on the GPU: a plan compiled with and
 without fusion gives the same amplitudes (complex64 tolerance) and the
-oracle's; the chain kernel really ran; bad programs are rejected on the
-host before anything is launched."""
+oracle's; the chain kernel really ran; the larger operator stages (16 -> 32,
+32 x 32, 4 -> 32, 2 -> 4; the complex products of up to 16 x 16
+take three real MFMAs, the larger ones four) against FP64 plans and the
+committed 5x5 depth-12 amplitudes; bad programs are rejected on the host
+before anything is launched."""
+import json
+
 import numpy as np
 import pytest
 
 from oracle import pq_oracle as O
 import picoquant_hip as P
 from picoquant_hip import _lib, circuits as C, planner as PL
-from picoquant_hip.executor import compile_plan
+from picoquant_hip.executor import chain_stages, compile_plan
 
 pytestmark = pytest.mark.gpu
 
@@ -61,6 +66,62 @@ def test_fused_plan_matches_unfused_and_oracle(tile_bits):
             assert fused.step_info(i)["kc"] == _lib.KC_CHAIN
     base.close()
     fused.close()
+
+
+def _shapes(p):
+    out = set()
+    for i, st in enumerate(p.steps):
+        if st[0] == 2:
+            assert p.step_info(i)["kc"] == _lib.KC_CHAIN
+            out |= set(chain_stages(p.chain_programs[i]))
+    return out
+
+
+def test_large_operator_stages_against_fp64():
+    """5x5 depth 12 (seed 33, last qubit open), reconfigured column sweep,
+    2^13-element tiles and operators of up to 32 x 32: chains with 2 -> 4,
+    4 -> 32 and 16 -> 32 stages, against the same plan in FP64."""
+    tn = P.network_from_gates(25, C.grid_circuit_gates(5, 5, 12, 33), backend=P.HIPBackend(np.complex128),
+                              decompose=True)
+    P.add_input(tn, "0" * 25)
+    P.add_output(tn, "0" * 24 + "x")
+    plan = PL.reconfigure(tn, PL.sweep_plan(tn, PL.grid_site_order(5, 5, "col")), windows=(8,))
+    limits = {"tile_bits": 13, "min_elems": 64, "op_elems": 1024}
+    fused = compile_plan(tn, plan, precision="mixed", small_elems=16, fuse=limits)
+    assert {(1, 2), (2, 5), (4, 5)} <= _shapes(fused)
+    ref = compile_plan(tn, plan)
+    ref.execute(True)
+    want = ref.result().reshape(-1, order="F")
+    ref.close()
+    for use_graph in (False, True, True):
+        fused.execute(use_graph)
+        got = fused.result().reshape(-1, order="F")
+        assert np.linalg.norm(got - want) / np.linalg.norm(want) < 2e-6
+    fused.close()
+
+
+def test_32x32_stage_against_golden(golden_dir):
+    """Config 2 (5x5 depth 12, its committed batch of 16 amplitudes) on the
+    reconfigured column sweep with the executor's default fusion (2^13-
+    element tiles, operators up to 32 x 32, tensors from 2^16 elements):
+    chains with a 32 x 32 stage, within 1e-5 of the committed amplitudes."""
+    g = json.load(open(golden_dir / "grid5x5_d12.json"))
+    cfg = list(g["batch_base"])
+    for q in g["open_qubits_1based"]:
+        cfg[q - 1] = "x"
+    tn = P.network_from_gates(25, C.grid_circuit_gates(5, 5, 12, g["seed"]), backend=P.HIPBackend(np.complex128),
+                              decompose=True)
+    P.add_input(tn, "0" * 25)
+    P.add_output(tn, "".join(cfg))
+    plan = PL.reconfigure(tn, PL.sweep_plan(tn, PL.grid_site_order(5, 5, "col", open_last=g["open_qubits_1based"])))
+    p = compile_plan(tn, plan, precision="mixed", small_elems=1 << 16)
+    assert (5, 5) in _shapes(p)
+    want = np.array([complex(a, b) for a, b in g["batch_amplitudes"]])
+    for use_graph in (False, True):
+        p.execute(use_graph)
+        got = p.result().reshape(-1, order="F")
+        assert float(np.max(np.abs(got - want) / np.abs(want))) < 1e-5
+    p.close()
 
 
 def test_pair_moves_do_not_change_results():
