@@ -194,7 +194,9 @@ __device__ __forceinline__ void stage(lc* tile, const lc* sd, const int* si_, co
 
 // GV / SV: the gather / scatter moves each element pair as one 16-byte
 // access (a.gvec / a.svec); separate instantiations keep the two address
-// forms out of each other's code.
+// forms out of each other's code.  Four waves per SIMD (two workgroups per
+// CU) is the design point: the register allocator is held to 128 VGPRs
+// rather than trading the second workgroup for registers.
 template <bool GV, bool SV>
 __global__ __launch_bounds__(CH_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_chain(const ChainArgs a) {
   extern __shared__ lc smem[];
