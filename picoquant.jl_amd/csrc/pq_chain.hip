@@ -346,7 +346,8 @@ __global__ __launch_bounds__(CH_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4
   // both in their memory phase at once; the first half of the grid (one
   // workgroup of each pair when the dispatcher fills every CU once, then
   // again) issues at a higher wave priority, so its stages run ahead and
-  // its memory phase overlaps the other's stages (about 2 % per chain).
+  // its memory phase overlaps the other's stages (2 % per chain when the
+  // loads waited for the previous tile's stores; neutral since).
   if (blockIdx.x < G / 2) __builtin_amdgcn_s_setprio(3);
   int32_t cx = 0, cy = 0, nx = 0, ny = 0;   // offsets of the current and the next tile
   if (t < total) {
