@@ -384,8 +384,27 @@ __global__ __launch_bounds__(256) void k_forest(const ForestStep* __restrict__ f
     const int32_t i0 = bnd[l] + (int32_t)threadIdx.x, i1 = bnd[l + 1];
     const int4 cur = nx;
     if (l + 1 < nlev && i1 + (int32_t)threadIdx.x < bnd[l + 2]) nx = items[i1 + threadIdx.x];
-    for (int32_t i = i0; i < i1; i += (int32_t)blockDim.x) {
-      const int4 it = i == i0 ? cur : items[i];
+    // the lane's next items of this level, loaded together (a level of a
+    // few thousand outputs gives a lane several)
+    constexpr int PRE = 3;
+    int4 pre[PRE];
+#pragma unroll
+    for (int j = 0; j < PRE; ++j) {
+      const int32_t i = i0 + (j + 1) * (int32_t)blockDim.x;
+      if (i < i1) pre[j] = items[i];
+    }
+    int32_t c = 0;
+    for (int32_t i = i0; i < i1; i += (int32_t)blockDim.x, ++c) {
+      int4 it;
+      if (c == 0) {
+        it = cur;
+      } else if (c <= PRE) {
+        it = pre[0];
+        if (c == 2) it = pre[1];
+        if (c == 3) it = pre[2];
+      } else {
+        it = items[i];
+      }
       const ForestStep& f = F[it.x - fs0];
       if (f.c128)
         forest_item<c128>(f, it, KT + (f.kt - k0), dat);
