@@ -192,12 +192,81 @@ __device__ __forceinline__ void stage(lc* tile, const lc* sd, const int* si_, co
   }
 }
 
+// Stages of at most 4 x 4 (K, M <= 4) with at least 64 R values (the host
+// then takes 6 R bits from the lane): each lane owns one r (the lane's R
+// table entry XOR its group's) and runs the whole product on
+// v_mfma_f32_4x4x1f32, whose 16 blocks of 4 lanes take 4 r's each: lane
+// 4b + j supplies T[r, k] as block b's B column j and S[j, k] as its A row
+// j (the same operator in every block), and receives its r's M values in
+// its 4 accumulator registers.  Three real products per complex one
+// (Gauss), one instruction per K value: no padded MFMA rows, where a
+// 16 x 16 x 4 block would leave 12 of its 16 rows unused.
+__device__ __forceinline__ floatx4 mfma4(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ void stage_rows(lc* tile, const lc* sd, const int* si_, const ChainStage& g, int wave,
+                                           int lane) {
+  const int nk = 1 << g.nk, nm = 1 << g.nm;
+  const int m = lane & 3;
+  float sr[4], si[4];
+  int kof[4], mof[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    lc v;
+    v.x = 0.f;
+    v.y = 0.f;
+    if (k < nk && m < nm) v = sd[g.sd + k * nm + m];
+    sr[k] = v.x;
+    si[k] = v.y;
+    kof[k] = k < nk ? si_[g.ko + k] : 0;
+    mof[k] = k < nm ? si_[g.mo + k] : 0;
+  }
+  const int rlo = si_[g.ro + lane];
+  const int ng = 1 << g.rhi;
+  constexpr int NW = CH_THREADS / 64;
+  for (int grp = wave; grp < ng; grp += NW) {
+    const int roff = rlo ^ si_[g.rho + grp];
+    float tr[4], ti[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      tr[k] = 0.f;
+      ti[k] = 0.f;
+      if (k < nk) {
+        const lc v = at(tile, roff ^ kof[k]);
+        tr[k] = v.x;
+        ti[k] = v.y;
+      }
+    }
+    floatx4 p1 = {0.f, 0.f, 0.f, 0.f}, p2 = {0.f, 0.f, 0.f, 0.f}, p3 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k < nk) {
+        p1 = mfma4(sr[k], tr[k], p1);
+        p2 = mfma4(si[k], ti[k], p2);
+        p3 = mfma4(sr[k] + si[k], tr[k] + ti[k], p3);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i < nm) {
+        float* e = &at(tile, roff ^ mof[i]).x;
+        e[0] = p1[i] - p2[i];
+        e[1] = (p3[i] - p1[i]) - p2[i];
+      }
+    }
+  }
+}
+
 // GV / SV: the gather / scatter moves each element pair as one 16-byte
 // access (a.gvec / a.svec); separate instantiations keep the two address
-// forms out of each other's code.  Four waves per SIMD (two workgroups per
-// CU) is the design point: the register allocator is held to 128 VGPRs
-// rather than trading the second workgroup for registers.
-template <bool GV, bool SV>
+// forms out of each other's code.  SMALL: every stage's operator has at
+// most 4 chunk-blocks (the three-product stages), so only those stage loops
+// and the lane-row 4 x 4 stage are compiled in (the register allocation of
+// the whole kernel follows the stages it holds; the lane-row stage fits
+// only here without spills).  Four waves per SIMD (two workgroups per CU)
+// is the design point: the register allocator is held to 128 VGPRs rather
+// than trading the second workgroup for registers.
+template <bool GV, bool SV, bool SMALL>
 __global__ __launch_bounds__(CH_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_chain(const ChainArgs a) {
   extern __shared__ lc smem[];
   const int tid = threadIdx.x, lane = tid & 63;
@@ -379,22 +448,42 @@ __global__ __launch_bounds__(CH_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4
       const int nkc = ((1 << g.nk) + 3) / 4, nmb = ((1 << g.nm) + 15) / 16;
       // FULL: 16 low R values (rlo = 4) and every M row of the blocks used
       const bool full = g.rlo == 4 && g.nm >= 4 + (nmb == 2);
-      switch ((nkc * 4 + nmb) * 2 + (full ? 1 : 0)) {
-        case (4 * 4 + 1) * 2 + 1: stage<4, 1, true>(tile, sd, sidx, g, wave, li, lg); break;
-        case (2 * 4 + 2) * 2 + 1: stage<2, 2, true>(tile, sd, sidx, g, wave, li, lg); break;
-        case (1 * 4 + 1) * 2:
-        case (1 * 4 + 1) * 2 + 1: stage<1, 1, false>(tile, sd, sidx, g, wave, li, lg); break;
-        case (2 * 4 + 1) * 2:
-        case (2 * 4 + 1) * 2 + 1: stage<2, 1, false>(tile, sd, sidx, g, wave, li, lg); break;
-        case (4 * 4 + 1) * 2: stage<4, 1, false>(tile, sd, sidx, g, wave, li, lg); break;
-        case (8 * 4 + 1) * 2:
-        case (8 * 4 + 1) * 2 + 1: stage<8, 1, false>(tile, sd, sidx, g, wave, li, lg); break;
-        case (1 * 4 + 2) * 2:
-        case (1 * 4 + 2) * 2 + 1: stage<1, 2, false>(tile, sd, sidx, g, wave, li, lg); break;
-        case (2 * 4 + 2) * 2: stage<2, 2, false>(tile, sd, sidx, g, wave, li, lg); break;
-        case (4 * 4 + 2) * 2:
-        case (4 * 4 + 2) * 2 + 1: stage<4, 2, false>(tile, sd, sidx, g, wave, li, lg); break;
-        default: stage<8, 2, false>(tile, sd, sidx, g, wave, li, lg); break;
+      if constexpr (SMALL) {
+        if (g.rlo == 6) {
+          stage_rows(tile, sd, sidx, g, wave, lane);
+          __syncthreads();
+          continue;
+        }
+        switch ((nkc * 4 + nmb) * 2 + (full ? 1 : 0)) {
+          case (4 * 4 + 1) * 2 + 1: stage<4, 1, true>(tile, sd, sidx, g, wave, li, lg); break;
+          case (2 * 4 + 2) * 2 + 1: stage<2, 2, true>(tile, sd, sidx, g, wave, li, lg); break;
+          case (1 * 4 + 1) * 2:
+          case (1 * 4 + 1) * 2 + 1: stage<1, 1, false>(tile, sd, sidx, g, wave, li, lg); break;
+          case (2 * 4 + 1) * 2:
+          case (2 * 4 + 1) * 2 + 1: stage<2, 1, false>(tile, sd, sidx, g, wave, li, lg); break;
+          case (4 * 4 + 1) * 2: stage<4, 1, false>(tile, sd, sidx, g, wave, li, lg); break;
+          case (1 * 4 + 2) * 2:
+          case (1 * 4 + 2) * 2 + 1: stage<1, 2, false>(tile, sd, sidx, g, wave, li, lg); break;
+          default: stage<2, 2, false>(tile, sd, sidx, g, wave, li, lg); break;
+        }
+      } else {
+        switch ((nkc * 4 + nmb) * 2 + (full ? 1 : 0)) {
+          case (4 * 4 + 1) * 2 + 1: stage<4, 1, true>(tile, sd, sidx, g, wave, li, lg); break;
+          case (2 * 4 + 2) * 2 + 1: stage<2, 2, true>(tile, sd, sidx, g, wave, li, lg); break;
+          case (1 * 4 + 1) * 2:
+          case (1 * 4 + 1) * 2 + 1: stage<1, 1, false>(tile, sd, sidx, g, wave, li, lg); break;
+          case (2 * 4 + 1) * 2:
+          case (2 * 4 + 1) * 2 + 1: stage<2, 1, false>(tile, sd, sidx, g, wave, li, lg); break;
+          case (4 * 4 + 1) * 2: stage<4, 1, false>(tile, sd, sidx, g, wave, li, lg); break;
+          case (8 * 4 + 1) * 2:
+          case (8 * 4 + 1) * 2 + 1: stage<8, 1, false>(tile, sd, sidx, g, wave, li, lg); break;
+          case (1 * 4 + 2) * 2:
+          case (1 * 4 + 2) * 2 + 1: stage<1, 2, false>(tile, sd, sidx, g, wave, li, lg); break;
+          case (2 * 4 + 2) * 2: stage<2, 2, false>(tile, sd, sidx, g, wave, li, lg); break;
+          case (4 * 4 + 2) * 2:
+          case (4 * 4 + 2) * 2 + 1: stage<4, 2, false>(tile, sd, sidx, g, wave, li, lg); break;
+          default: stage<8, 2, false>(tile, sd, sidx, g, wave, li, lg); break;
+        }
       }
       __syncthreads();
     }
@@ -432,18 +521,38 @@ hipError_t launch_chain(const ChainArgs& a, hipStream_t s) {
     if (e != hipSuccess) return e;
     if (ncu[dev] < 1) ncu[dev] = 1;
   }
+  // the small-stage instantiation for chains whose every operator has at
+  // most 4 chunk-blocks; lane-row stages (6 R bits from the lane) exist
+  // only in it
+  bool small = true, rows = false;
+  for (int t = 0; t < a.nst; ++t) {
+    const ChainStage& g = a.st[t];
+    const int nkc = ((1 << g.nk) + 3) / 4, nmb = ((1 << g.nm) + 15) / 16;
+    small = small && nkc * nmb <= 4;
+    if (g.rlo == 6) rows = true;
+    if (g.rlo < 0 || g.rlo > 6 || (g.rlo == 6 && (g.nk > 2 || g.nm > 2)) || (g.rlo == 5)) return hipErrorInvalidValue;
+  }
+  if (rows && !small) return hipErrorInvalidValue;
   int per_cu = 0;
-  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_chain<true, true>, CH_THREADS, lds);
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_chain<true, true, false>, CH_THREADS, lds);
   if (e != hipSuccess) return e;
   per_cu = std::max(per_cu, 1);
   const uint32_t grid = std::min<uint32_t>(a.spec.total, (uint32_t)(per_cu * ncu[dev]));
+#define PQ_CHAIN_LAUNCH(G, S)                                                             \
+  do {                                                                                    \
+    if (small)                                                                            \
+      hipLaunchKernelGGL((k_chain<G, S, true>), dim3(grid), dim3(CH_THREADS), lds, s, a);  \
+    else                                                                                  \
+      hipLaunchKernelGGL((k_chain<G, S, false>), dim3(grid), dim3(CH_THREADS), lds, s, a); \
+  } while (0)
   if (a.gvec && a.svec)
-    hipLaunchKernelGGL((k_chain<true, true>), dim3(grid), dim3(CH_THREADS), lds, s, a);
+    PQ_CHAIN_LAUNCH(true, true);
   else if (a.gvec)
-    hipLaunchKernelGGL((k_chain<true, false>), dim3(grid), dim3(CH_THREADS), lds, s, a);
+    PQ_CHAIN_LAUNCH(true, false);
   else if (a.svec)
-    hipLaunchKernelGGL((k_chain<false, true>), dim3(grid), dim3(CH_THREADS), lds, s, a);
+    PQ_CHAIN_LAUNCH(false, true);
   else
-    hipLaunchKernelGGL((k_chain<false, false>), dim3(grid), dim3(CH_THREADS), lds, s, a);
+    PQ_CHAIN_LAUNCH(false, false);
+#undef PQ_CHAIN_LAUNCH
   return hipGetLastError();
 }

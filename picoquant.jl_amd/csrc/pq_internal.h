@@ -143,7 +143,7 @@ int reduce_blocks_for(uint32_t ktotal);
 struct ChainStage {
   const c64* S;
   int32_t nk, nm;          // 2^nk operator columns, 2^nm rows
-  int32_t rlo, rhi;        // R bits from the lane (<= 4) and from the block counter
+  int32_t rlo, rhi;        // R bits from the lane (<= 4; 6 for K, M <= 4) and from the block counter
   int32_t kp, mp, sk, sm;  // table offsets: K / M tile addresses, K / M operator offsets
   int32_t rl, rh;          // table offsets: R tile addresses (low, high part)
   int32_t sd, ko, mo, ro, rho;  // LDS staging: dense operator [m][k] (c64 units), K / M /

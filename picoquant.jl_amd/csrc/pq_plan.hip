@@ -559,7 +559,7 @@ const char* parse_chain(pq_ctx* c, pq_plan* p, const pq_plan_desc* d, int i, PSt
     if (!slot_holds(S, sspan + shigh)) return "operator access out of range";
     g.nk = nk;
     g.nm = nm;
-    g.rlo = std::min(nr, 4);
+    g.rlo = std::min(nr, 4);   // R bits from the lane (16 r's per MFMA block)
     g.rhi = nr - g.rlo;
     kps[t] = kpos;
     kss[t] = kstr;
@@ -579,6 +579,41 @@ const char* parse_chain(pq_ctx* c, pq_plan* p, const pq_plan_desc* d, int i, PSt
     p->slots[os].last_use = i;
     st.flops += 8.0 * (double)((int64_t)1 << (nr + nk + nm)) * (double)a.spec.total;
     st.bytes += 8.0 * (double)S.elems;
+  }
+  // A chain whose every operator has at most 4 chunk-blocks runs on the
+  // kernel's small-stage form, which also has the lane-row stage: operators
+  // of at most 4 x 4 with at least 64 r's take 6 R bits from the lane (one
+  // r per lane on 4 x 4 x 1 MFMAs), when the larger R tables still fit the
+  // LDS staging area.
+  {
+    bool small_chain = true;
+    for (int t = 0; t < L; ++t) {
+      const int nkc = ((1 << a.st[t].nk) + 3) / 4, nmb = ((1 << a.st[t].nm) + 15) / 16;
+      small_chain = small_chain && nkc * nmb <= 4;
+    }
+    auto rows = [&](const ChainStage& g) { return g.nk <= 2 && g.nm <= 2 && g.rlo + g.rhi >= 6; };
+    int32_t idn = 0;
+    for (int t = 0; t < L; ++t) {
+      const ChainStage& g = a.st[t];
+      const int nr = g.rlo + g.rhi, rl = small_chain && rows(g) ? 6 : g.rlo;
+      idn += (1 << g.nk) + (1 << g.nm) + (1 << rl) + (1 << (nr - rl));
+    }
+    if (small_chain && (int64_t)a.sdn * 8 + (int64_t)idn * 4 <= CH_STAGE_BYTES) {
+      a.idn = 0;
+      for (int t = 0; t < L; ++t) {
+        ChainStage& g = a.st[t];
+        const int nr = g.rlo + g.rhi;
+        if (rows(g)) {
+          g.rlo = 6;
+          g.rhi = nr - 6;
+        }
+        g.ko = a.idn;
+        g.mo = a.idn + (1 << g.nk);
+        g.ro = g.mo + (1 << g.nm);
+        g.rho = g.ro + (1 << g.rlo);
+        a.idn += (1 << g.nk) + (1 << g.nm) + (1 << g.rlo) + (1 << g.rhi);
+      }
+    }
   }
   // tile layout for the accesses' lane bits, then the address tables
   std::vector<std::vector<int32_t>> sets;
