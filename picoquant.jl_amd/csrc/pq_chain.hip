@@ -64,14 +64,16 @@ __device__ __forceinline__ floatx4 mfma(float a, float b, floatx4 c) {
 // row m = 16 mb + li, column k = 4 kc + lg) stay in registers when there
 // are at most 8 chunk-blocks of them; the 32 x 32 case rereads them from
 // LDS per M block (register room for the next tile's prefetch).  With at
-// most 4 chunk-blocks (16 x 16 and smaller, 8 -> 32) a complex product takes
-// three real MFMAs (Gauss), otherwise four: the third accumulator and the
-// sums of the larger operators cost registers, and measured slower.
-template <int NKC, int NMB, bool FULL>
+// most 4 chunk-blocks (16 x 16 and smaller, 8 -> 32), and with G8 (the
+// one-M-block kernel form) the 32-column ones, a complex product takes
+// three real MFMAs (Gauss), otherwise four: in the full kernel form the
+// third accumulator and the sums of the larger operators cost registers,
+// and measured slower.
+template <int NKC, int NMB, bool FULL, bool G8 = false>
 __device__ __forceinline__ void stage(lc* tile, const lc* sd, const int* si_, const ChainStage& g, int wave, int li,
                                       int lg) {
   constexpr bool HOLD = NKC * NMB <= 8;
-  constexpr bool GAUSS = NKC * NMB <= 4;
+  constexpr bool GAUSS = NKC * NMB <= 4 || G8;
   constexpr int HMB = HOLD ? NMB : 1;
   const int nk = 1 << g.nk, nm = 1 << g.nm;
   auto opval = [&](int kc, int mb) -> lc {
@@ -259,14 +261,17 @@ __device__ __forceinline__ void stage_rows(lc* tile, const lc* sd, const int* si
 
 // GV / SV: the gather / scatter moves each element pair as one 16-byte
 // access (a.gvec / a.svec); separate instantiations keep the two address
-// forms out of each other's code.  SMALL: every stage's operator has at
+// forms out of each other's code.  MODE 1: every stage's operator has at
 // most 4 chunk-blocks (the three-product stages), so only those stage loops
-// and the lane-row 4 x 4 stage are compiled in (the register allocation of
-// the whole kernel follows the stages it holds; the lane-row stage fits
-// only here without spills).  Four waves per SIMD (two workgroups per CU)
-// is the design point: the register allocator is held to 128 VGPRs rather
-// than trading the second workgroup for registers.
-template <bool GV, bool SV, bool SMALL>
+// and the lane-row 4 x 4 stage are compiled in; MODE 2: every operator has
+// one M block (up to 32 -> 16), and the 32-column stages take three
+// products too; MODE 0: every stage form.  The register allocation of the
+// whole kernel follows the stages it holds: the lane-row stage and the
+// three-product 32 -> 8 stage fit only in the leaner forms without spills
+// or slower code.  Four waves per SIMD (two workgroups per CU) is the
+// design point: the register allocator is held to 128 VGPRs rather than
+// trading the second workgroup for registers.
+template <bool GV, bool SV, int MODE>
 __global__ __launch_bounds__(CH_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_chain(const ChainArgs a) {
   extern __shared__ lc smem[];
   const int tid = threadIdx.x, lane = tid & 63;
@@ -448,7 +453,17 @@ __global__ __launch_bounds__(CH_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4
       const int nkc = ((1 << g.nk) + 3) / 4, nmb = ((1 << g.nm) + 15) / 16;
       // FULL: 16 low R values (rlo = 4) and every M row of the blocks used
       const bool full = g.rlo == 4 && g.nm >= 4 + (nmb == 2);
-      if constexpr (SMALL) {
+      if constexpr (MODE == 2) {
+        switch ((nkc * 4 + nmb) * 2 + (full ? 1 : 0)) {
+          case (4 * 4 + 1) * 2 + 1: stage<4, 1, true>(tile, sd, sidx, g, wave, li, lg); break;
+          case (1 * 4 + 1) * 2:
+          case (1 * 4 + 1) * 2 + 1: stage<1, 1, false>(tile, sd, sidx, g, wave, li, lg); break;
+          case (2 * 4 + 1) * 2:
+          case (2 * 4 + 1) * 2 + 1: stage<2, 1, false>(tile, sd, sidx, g, wave, li, lg); break;
+          case (4 * 4 + 1) * 2: stage<4, 1, false>(tile, sd, sidx, g, wave, li, lg); break;
+          default: stage<8, 1, false, true>(tile, sd, sidx, g, wave, li, lg); break;
+        }
+      } else if constexpr (MODE == 1) {
         if (g.rlo == 6) {
           stage_rows(tile, sd, sidx, g, wave, lane);
           __syncthreads();
@@ -522,28 +537,32 @@ hipError_t launch_chain(const ChainArgs& a, hipStream_t s) {
     if (ncu[dev] < 1) ncu[dev] = 1;
   }
   // the small-stage instantiation for chains whose every operator has at
-  // most 4 chunk-blocks; lane-row stages (6 R bits from the lane) exist
-  // only in it
-  bool small = true, rows = false;
+  // most 4 chunk-blocks (lane-row stages, 6 R bits from the lane, exist
+  // only in it), the one-M-block instantiation for chains whose every
+  // operator has one M block, else the full one
+  bool small = true, rows = false, col = true;
   for (int t = 0; t < a.nst; ++t) {
     const ChainStage& g = a.st[t];
     const int nkc = ((1 << g.nk) + 3) / 4, nmb = ((1 << g.nm) + 15) / 16;
     small = small && nkc * nmb <= 4;
+    col = col && nmb == 1;
     if (g.rlo == 6) rows = true;
     if (g.rlo < 0 || g.rlo > 6 || (g.rlo == 6 && (g.nk > 2 || g.nm > 2)) || (g.rlo == 5)) return hipErrorInvalidValue;
   }
   if (rows && !small) return hipErrorInvalidValue;
   int per_cu = 0;
-  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_chain<true, true, false>, CH_THREADS, lds);
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_chain<true, true, 0>, CH_THREADS, lds);
   if (e != hipSuccess) return e;
   per_cu = std::max(per_cu, 1);
   const uint32_t grid = std::min<uint32_t>(a.spec.total, (uint32_t)(per_cu * ncu[dev]));
-#define PQ_CHAIN_LAUNCH(G, S)                                                             \
-  do {                                                                                    \
-    if (small)                                                                            \
-      hipLaunchKernelGGL((k_chain<G, S, true>), dim3(grid), dim3(CH_THREADS), lds, s, a);  \
-    else                                                                                  \
-      hipLaunchKernelGGL((k_chain<G, S, false>), dim3(grid), dim3(CH_THREADS), lds, s, a); \
+#define PQ_CHAIN_LAUNCH(G, S)                                                         \
+  do {                                                                                \
+    if (small)                                                                        \
+      hipLaunchKernelGGL((k_chain<G, S, 1>), dim3(grid), dim3(CH_THREADS), lds, s, a); \
+    else if (col)                                                                     \
+      hipLaunchKernelGGL((k_chain<G, S, 2>), dim3(grid), dim3(CH_THREADS), lds, s, a); \
+    else                                                                              \
+      hipLaunchKernelGGL((k_chain<G, S, 0>), dim3(grid), dim3(CH_THREADS), lds, s, a); \
   } while (0)
   if (a.gvec && a.svec)
     PQ_CHAIN_LAUNCH(true, true);
